@@ -1,0 +1,140 @@
+/*
+ * pfsoft_amd.h -- C ABI of the MI355X implementation of the discrete SO(3)
+ * Fourier transform (DSOFT) and its inverse (iDSOFT).
+ *
+ * Plain C: pointers, sizes and int status codes.  No torch or HIP types cross
+ * this boundary (streams are passed as void*, a hipStream_t or NULL).
+ *
+ * Layouts (N = 2B, complex128 = two interleaved doubles):
+ *   grid    N^3 complex, element (row r, col c, slice k) at N*N*k + N*c + r --
+ *           the reference's grid3D<complex<double>> memory
+ *           (include/PFSOFTlib_headers/grid3D_cx.hpp:293-303).
+ *   coeffs  B(4B^2-1)/3 complex, packed by degree: block l starts at
+ *           (4l^3 - l)/3; element (l, M, M') at that offset + iM + (2l+1)*iM',
+ *           iX = X >= 0 ? X : 2l+1+X -- the reference's per-degree column-major
+ *           matrices laid end to end (src/dsoft_fourier_coefficients.cpp:48-112).
+ *
+ * Every function returns PFS_OK (0) or a PFS_ERR_* code; pfs_last_error()
+ * gives the message of the last failure on the calling thread.  Calls on one
+ * context are serialised by the context; separate contexts may run
+ * concurrently.
+ */
+#ifndef PFSOFT_AMD_H
+#define PFSOFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pfs_context pfs_context;
+
+enum pfs_status {
+    PFS_OK = 0,
+    PFS_ERR_ARG = 1,        /* null pointer, bad size or bad enum value        */
+    PFS_ERR_SHAPE = 2,      /* grid not cubic / not even / wrong bandwidth      */
+    PFS_ERR_DEVICE = 3,     /* HIP runtime error                                */
+    PFS_ERR_MEMORY = 4,     /* device allocation failed                         */
+    PFS_ERR_UNSUPPORTED = 5,/* bandwidth or size outside the supported range    */
+    PFS_ERR_BUDGET = 6      /* request does not fit the context's HBM budget    */
+};
+
+enum pfs_precision {
+    PFS_F64 = 0,            /* FP64 throughout (the plain path)                 */
+    PFS_F32 = 1             /* FP32 FFT and DWT, FP64 recurrence tables         */
+};
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* pfs_last_error(void);
+
+/* Library version string. */
+const char* pfs_version(void);
+
+/* Context on one HIP device.  hbm_budget_bytes = 0 means no limit.
+ * Replaces the reference's implicit process-wide state (FFTW plans, OpenMP
+ * pool); there is no counterpart function in the reference. */
+int pfs_ctx_create(int device, size_t hbm_budget_bytes, pfs_context** out);
+int pfs_ctx_destroy(pfs_context* ctx);
+
+/* Build and keep resident the per-bandwidth tables (quadrature weights,
+ * cos/sin of the Chebyshev nodes, Wigner base norms, FFT twiddles).  Called
+ * implicitly by the transforms. */
+int pfs_plan_warm(pfs_context* ctx, int bandwidth);
+
+/* Number of coefficients B(4B^2-1)/3 and the packed index of (l, M, M'). */
+long pfs_coeff_count(int bandwidth);
+long pfs_coeff_index(int l, int M, int Mp);
+
+/* Forward transform, host buffers.  Replaces
+ * pfsoft::FourierTransforms::DSOFT(grid3D<complex<double>> sample,
+ *   DSOFTFourierCoefficients& fc, int threads)
+ *   (include/PFSOFTlib_headers/fn_fourier_transforms.hpp:75, src/fn_dsoft.cpp:76).
+ * grid: N^3 complex128 (read only -- the reference takes it by value);
+ * coeffs: pfs_coeff_count(B) complex128, written. */
+int pfs_dsoft_f64(pfs_context* ctx, int bandwidth, const double* grid, double* coeffs);
+
+/* Inverse transform, host buffers.  Replaces
+ * pfsoft::FourierTransforms::IDSOFT(const DSOFTFourierCoefficients& fc,
+ *   grid3D<complex<double>>& synthesis, int threads)
+ *   (fn_fourier_transforms.hpp:78, src/fn_idsoft.cpp:72).
+ * grid is read and written: as in the reference, the entries with row == B or
+ * col == B are never produced by the Wigner stage, so the inverse FFT reads
+ * what the caller's grid held there (zero for a fresh grid). */
+int pfs_idsoft_f64(pfs_context* ctx, int bandwidth, const double* coeffs, double* grid);
+
+/* Same transforms on device buffers (HBM-resident), enqueued on stream
+ * (NULL = the context's stream); the calls return once enqueued. */
+int pfs_dsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* grid_dev, void* coeffs_dev, void* stream);
+int pfs_idsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* grid_dev, void* stream);
+
+/* Layer-wise unnormalised 2D DFT of lays square slices, in place, host
+ * memory.  Replaces uzl_fftw_layer_wise_DFT2_grid3D / _IDFT2_grid3D
+ * (include/PFSOFTlib_headers/fftw_wrapper.hpp:28-33, src/fftw_wrapper.cpp:47-172):
+ * same arguments, same layout (n0 = cols slow, n1 = rows fast), forward sign
+ * -1 / backward +1, no normalisation.  threads is accepted and ignored.
+ * Only cols == rows is supported; failures are reported through
+ * pfs_last_error() since the reference signature returns void.  Uses a
+ * process-wide context on device 0. */
+void pfs_layer_wise_DFT2_grid3D(int cols, int rows, int lays, double* arr, int threads);
+void pfs_layer_wise_IDFT2_grid3D(int cols, int rows, int lays, double* arr, int threads);
+
+/* Device form of the seam: in place on arr_dev, result multiplied by scale
+ * (the reference's grid3D::layer_wise_DFT2/IDFT2 scale argument,
+ * grid3D_cx.hpp:309-427). sign -1 forward, +1 backward. */
+int pfs_layer_wise_dft2_dev(pfs_context* ctx, int n, int lays, void* arr_dev, int sign, double scale,
+                            void* stream);
+
+/* Public Wigner helpers on the device (include/PFSOFTlib_headers/fn_dwt.hpp:64,
+ * :107, :239).  out is host memory.
+ *   pfs_quadrature_weights: 2B values w_B(j).
+ *   pfs_wigner_d_matrix:   (B - max(|M|,|M'|)) x 2B, column-major like the
+ *                          reference matrix, rows l = max(|M|,|M'|) .. B-1,
+ *                          columns beta_k; weighted != 0 multiplies column k
+ *                          by w_B(k).  Values carry the reference's base sign
+ *                          -1 (fn_dwt.hpp:124-125).  Requires M >= M' >= 0,
+ *                          the only orders the transforms use. */
+int pfs_quadrature_weights(pfs_context* ctx, int bandwidth, double* out);
+int pfs_wigner_d_matrix(pfs_context* ctx, int bandwidth, int M, int Mp, int weighted, double* out);
+
+/* Pipeline stages on device buffers (for stage timing and stage parity).
+ * pencils: N^3 complex in the layout F(p, q, k) at (p*N + q)*N + k, the 2D
+ * spectrum of every beta-slice with the slice index fastest. */
+int pfs_stage_fft_fwd_dev(pfs_context* ctx, int bandwidth, const void* grid_dev, void* pencils_dev, void* stream);
+int pfs_stage_dwt_fwd_dev(pfs_context* ctx, int bandwidth, const void* pencils_dev, void* coeffs_dev, void* stream);
+int pfs_stage_dwt_inv_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* pencils_dev, void* stream);
+int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_dev, void* grid_dev, void* stream);
+
+/* Device memory helpers for callers without their own allocator. */
+int pfs_device_alloc(pfs_context* ctx, size_t bytes, void** out);
+int pfs_device_free(pfs_context* ctx, void* ptr);
+int pfs_memcpy_h2d(pfs_context* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int pfs_memcpy_d2h(pfs_context* ctx, void* dst_host, const void* src_dev, size_t bytes);
+int pfs_synchronize(pfs_context* ctx, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFSOFT_AMD_H */

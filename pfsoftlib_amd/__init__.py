@@ -1,0 +1,300 @@
+"""MI355X DSOFT / iDSOFT (discrete SO(3) Fourier transform) -- Python host side.
+
+The compute lives in libpfsoft_amd.so (hand-written HIP for gfx950) behind the
+C ABI of include/pfsoft_amd.h; this module binds it with ctypes and mirrors
+the reference's C++ interface for the path:
+
+    DSOFT(sample, fc, threads)       pfsoft::FourierTransforms::DSOFT
+                                     (include/PFSOFTlib_headers/fn_fourier_transforms.hpp:75)
+    IDSOFT(fc, synthesis, threads)   pfsoft::FourierTransforms::IDSOFT (:78)
+    DSOFTFourierCoefficients(B)      include/PFSOFTlib_headers/dsoft_fourier_coefficients.hpp:50
+    grid3D(n)                        grid3D<complex<double>> (grid3D_cx.hpp:47)
+
+There is no CPU fallback: loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+from .layout import coeff_count, coeff_index, coeff_offset, seeded_coeffs  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libpfsoft_amd.so")
+_lib = None
+
+PFS_OK = 0
+PFS_F64 = 0
+PFS_F32 = 1
+
+
+class PfsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pfsoft_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded C-ABI library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            raise ImportError("pfsoftlib_amd: %s is not built; run __graft_entry__.build()" % _LIBPATH)
+        L = ctypes.CDLL(_LIBPATH)
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+        sig = {
+            "pfs_last_error": (ctypes.c_char_p, []),
+            "pfs_version": (ctypes.c_char_p, []),
+            "pfs_ctx_create": (ip, [ip, ctypes.c_size_t, ctypes.POINTER(vp)]),
+            "pfs_ctx_destroy": (ip, [vp]),
+            "pfs_plan_warm": (ip, [vp, ip]),
+            "pfs_coeff_count": (ctypes.c_long, [ip]),
+            "pfs_coeff_index": (ctypes.c_long, [ip, ip, ip]),
+            "pfs_dsoft_f64": (ip, [vp, ip, dp, dp]),
+            "pfs_idsoft_f64": (ip, [vp, ip, dp, dp]),
+            "pfs_dsoft_f64_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_idsoft_f64_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_layer_wise_DFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
+            "pfs_layer_wise_IDFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
+            "pfs_layer_wise_dft2_dev": (ip, [vp, ip, ip, vp, ip, ctypes.c_double, vp]),
+            "pfs_quadrature_weights": (ip, [vp, ip, dp]),
+            "pfs_wigner_d_matrix": (ip, [vp, ip, ip, ip, ip, dp]),
+            "pfs_stage_fft_fwd_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_stage_dwt_fwd_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_stage_dwt_inv_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_stage_fft_inv_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_device_alloc": (ip, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
+            "pfs_device_free": (ip, [vp, vp]),
+            "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),
+            "pfs_memcpy_d2h": (ip, [vp, vp, vp, ctypes.c_size_t]),
+            "pfs_synchronize": (ip, [vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != PFS_OK:
+        raise PfsError(rc, lib().pfs_last_error().decode())
+
+
+def _dptr(a):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _as_f64(a, n_complex):
+    a = np.ascontiguousarray(a, dtype=np.complex128).reshape(-1)
+    if a.size != n_complex:
+        raise ValueError("expected %d complex values, got %d" % (n_complex, a.size))
+    return a
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)  # torch.cuda.Stream
+
+
+def _devptr(t):
+    return t if isinstance(t, int) else int(t.data_ptr())
+
+
+class Context:
+    """One device, its stream, the resident per-bandwidth tables and work buffers."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = ctypes.c_void_p()
+        _check(lib().pfs_ctx_create(device, hbm_budget, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pfs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def warm(self, B):
+        _check(lib().pfs_plan_warm(self._h, B))
+
+    # ---- host-buffer transforms (numpy complex128) ----------------------
+    def dsoft(self, B, grid):
+        """grid: (2B)^3 complex128 in reference layout -> packed coefficients."""
+        g = _as_f64(grid, (2 * B) ** 3)
+        out = np.empty(coeff_count(B), dtype=np.complex128)
+        _check(lib().pfs_dsoft_f64(self._h, B, _dptr(g.view(np.float64)), _dptr(out.view(np.float64))))
+        return out
+
+    def idsoft(self, B, coeffs, grid=None):
+        """Packed coefficients -> (2B)^3 grid.  grid, if given, supplies the
+        entries with row or column index B (as the reference's in-place call)."""
+        c = _as_f64(coeffs, coeff_count(B))
+        N = 2 * B
+        if grid is None:
+            g = np.zeros(N ** 3, dtype=np.complex128)
+        else:
+            g = np.array(grid, dtype=np.complex128, copy=True).reshape(-1)
+            if g.size != N ** 3:
+                raise ValueError("grid must hold (2B)^3 values")
+        _check(lib().pfs_idsoft_f64(self._h, B, _dptr(c.view(np.float64)), _dptr(g.view(np.float64))))
+        return g
+
+    # ---- device-buffer transforms (torch tensors or raw pointers) -------
+    def dsoft_dev(self, B, grid_dev, coeffs_dev, stream=None):
+        _check(lib().pfs_dsoft_f64_dev(self._h, B, _devptr(grid_dev), _devptr(coeffs_dev), _stream_handle(stream)))
+
+    def idsoft_dev(self, B, coeffs_dev, grid_dev, stream=None):
+        _check(lib().pfs_idsoft_f64_dev(self._h, B, _devptr(coeffs_dev), _devptr(grid_dev), _stream_handle(stream)))
+
+    def stage_fft_fwd(self, B, grid_dev, pencils_dev, stream=None):
+        _check(lib().pfs_stage_fft_fwd_dev(self._h, B, _devptr(grid_dev), _devptr(pencils_dev), _stream_handle(stream)))
+
+    def stage_dwt_fwd(self, B, pencils_dev, coeffs_dev, stream=None):
+        _check(lib().pfs_stage_dwt_fwd_dev(self._h, B, _devptr(pencils_dev), _devptr(coeffs_dev), _stream_handle(stream)))
+
+    def stage_dwt_inv(self, B, coeffs_dev, pencils_dev, stream=None):
+        _check(lib().pfs_stage_dwt_inv_dev(self._h, B, _devptr(coeffs_dev), _devptr(pencils_dev), _stream_handle(stream)))
+
+    def stage_fft_inv(self, B, pencils_dev, grid_dev, stream=None):
+        _check(lib().pfs_stage_fft_inv_dev(self._h, B, _devptr(pencils_dev), _devptr(grid_dev), _stream_handle(stream)))
+
+    def layer_wise_dft2_dev(self, n, lays, arr_dev, sign, scale=1.0, stream=None):
+        _check(lib().pfs_layer_wise_dft2_dev(self._h, n, lays, _devptr(arr_dev), sign, scale, _stream_handle(stream)))
+
+    def synchronize(self, stream=None):
+        _check(lib().pfs_synchronize(self._h, _stream_handle(stream)))
+
+    # ---- public Wigner helpers (fn_dwt.hpp:64,107,239) ------------------
+    def quadrature_weights(self, B):
+        out = np.empty(2 * B, dtype=np.float64)
+        _check(lib().pfs_quadrature_weights(self._h, B, _dptr(out)))
+        return out
+
+    def wigner_d_matrix(self, B, M, Mp, weighted=False):
+        R = B - max(abs(M), abs(Mp))
+        out = np.empty(R * 2 * B, dtype=np.float64)
+        _check(lib().pfs_wigner_d_matrix(self._h, B, M, Mp, 1 if weighted else 0, _dptr(out)))
+        return out.reshape(2 * B, R).T.copy()
+
+
+def layer_wise_DFT2_grid3D(arr, inverse=False):
+    """In-place layer-wise 2D DFT of a complex128 array shaped (lays, cols, rows)
+    (src/fftw_wrapper.cpp:47-172).  Returns the array."""
+    a = np.ascontiguousarray(arr, dtype=np.complex128)
+    lays, cols, rows = a.shape
+    f = lib().pfs_layer_wise_IDFT2_grid3D if inverse else lib().pfs_layer_wise_DFT2_grid3D
+    f(cols, rows, lays, _dptr(a.view(np.float64).reshape(-1)), 1)
+    err = lib().pfs_last_error().decode()
+    if err:
+        raise PfsError(-1, err)
+    return a
+
+
+# --------------------------------------------------------------------------
+# Mirror of the reference's API types and functions.
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _warn(msg):
+    # same text shape as pfsoft_cond_w_ret (cmake_aux/compiler_config.hpp.in:112-120)
+    f = sys._getframe(2)
+    sys.stderr.write("** [PFSOFTLib warning] %s:%d:%s(): %s **\n" % (
+        os.path.basename(f.f_code.co_filename), f.f_lineno, f.f_code.co_name, msg))
+
+
+class DSOFTFourierCoefficients:
+    """Coefficients f^l_{M,M'} for 0 <= l < bandwidth, |M|,|M'| <= l, packed by
+    degree (dsoft_fourier_coefficients.cpp:48-112).  fc(l, M, Mp) is the
+    element; negative orders count from the end of the degree block."""
+
+    def __init__(self, bandwidth):
+        self.bandwidth = int(bandwidth)
+        self.mem = np.zeros(coeff_count(self.bandwidth), dtype=np.complex128)
+
+    def _idx(self, l, M, Mp):
+        if M > l or Mp > l or M < -l or Mp < -l or l < 0 or l >= self.bandwidth:
+            # the reference prints and calls exit(EXIT_FAILURE) (dsoft_fourier_coefficients.cpp:82)
+            raise IndexError("illegal parameter for DSOFTFourierCoefficients. Condition |M|,|Mp| <= l violated.")
+        return coeff_index(l, M, Mp)
+
+    def __call__(self, l, M, Mp):
+        return self.mem[self._idx(l, M, Mp)]
+
+    def set(self, l, M, Mp, value):
+        self.mem[self._idx(l, M, Mp)] = value
+
+
+class grid3D:
+    """rows x cols x lays complex128 grid, element (row, col, lay) at
+    rows*cols*lay + cols*col + row (grid3D_cx.hpp:293-303)."""
+
+    def __init__(self, rows, cols=None, lays=None):
+        self.rows = int(rows)
+        self.cols = int(cols if cols is not None else rows)
+        self.lays = int(lays if lays is not None else rows)
+        self.mem = np.zeros(self.rows * self.cols * self.lays, dtype=np.complex128)
+
+    def __call__(self, row, col, lay):
+        return self.mem[self.rows * self.cols * lay + self.cols * col + row]
+
+    def set(self, row, col, lay, value):
+        self.mem[self.rows * self.cols * lay + self.cols * col + row] = value
+
+
+def DSOFT(sample, fc, threads=None, ctx=None):
+    """Forward transform (src/fn_dsoft.cpp:76).  sample is not modified."""
+    if sample.rows != sample.cols or sample.rows != sample.lays:
+        return _warn("all DSOFT sample grid dimensions should be equal.")
+    if sample.rows & 1:
+        return _warn("DSOFT sample grid dimensions are not even.")
+    B = sample.cols // 2
+    if B != fc.bandwidth:
+        return _warn("DSOFT Fourier coefficients container bandwidth does not match to sample grid bandwidth.")
+    fc.mem[:] = (ctx or default_context()).dsoft(B, sample.mem)
+
+
+def IDSOFT(fc, synthesis, threads=None, ctx=None):
+    """Inverse transform (src/fn_idsoft.cpp:72), written into synthesis."""
+    if synthesis.rows != synthesis.cols or synthesis.rows != synthesis.lays:
+        return _warn("all IDSOFT synthesis grid dimensions should be equal.")
+    if synthesis.rows & 1:
+        return _warn("IDSOFT synthesis grid dimensions are not even.")
+    B = synthesis.cols // 2
+    if B != fc.bandwidth:
+        return _warn("IDSOFT Fourier coefficients container bandwidth does not match to synthesis grid bandwidth.")
+    synthesis.mem[:] = (ctx or default_context()).idsoft(B, fc.mem, synthesis.mem)
+
+
+__all__ = ["Context", "PfsError", "DSOFT", "IDSOFT", "DSOFTFourierCoefficients", "grid3D",
+           "coeff_count", "coeff_index", "coeff_offset", "seeded_coeffs", "layer_wise_DFT2_grid3D", "lib"]

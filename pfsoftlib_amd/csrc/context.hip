@@ -1,0 +1,759 @@
+// Context, per-bandwidth plans and the C ABI (include/pfsoft_amd.h).
+//
+// A context owns one HIP stream on one device, the resident per-bandwidth
+// tables and the work buffers of the single-signal pipeline:
+//   forward  grid --FFT rows--> W --FFT cols--> pencils G --DWT--> coeffs
+//   inverse  coeffs --iDWT--> G --IFFT cols--> W --IFFT rows, x 1/(4B^2)--> grid
+// Plan tables are computed on the device in FP64: the Chebyshev nodes, the
+// quadrature weights w_B (fn_dwt.hpp:62-83, compensated summation), the base
+// norms sqrt((2M+1)/2) sqrt(C(2M, M-M')) (fn_dwt.hpp:117-121) and the FFT
+// twiddles.
+#include "pfs_internal.h"
+#include "../../include/pfsoft_amd.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#include <cstdio>
+#include <cstdarg>
+#include <cstring>
+#include <cmath>
+
+using namespace pfs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(e_ == hipErrorOutOfMemory ? PFS_ERR_MEMORY : PFS_ERR_DEVICE,       \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));                    \
+    } while (0)
+
+constexpr int MAX_BANDWIDTH = 512;
+
+__global__ void k_plan_nodes(int B, double* cosb, double* sinh, double* cosh, double* wts, double2* tw, float2* tw32)
+{
+    const int N = 2 * B;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    double x = (2.0 * k + 1.0) / (4.0 * B);  // beta_k / pi
+    cosb[k] = cospi(x);
+    sinh[k] = sinpi(0.5 * x);
+    cosh[k] = cospi(0.5 * x);
+    // w_B(i) = 2/B sin(pi(2i+1)/(4B)) sum_{m<B} sin((2i+1)(2m+1) pi/(4B)) / (2m+1)
+    int i = k < B ? k : N - 1 - k;
+    double s = 0.0, comp = 0.0;
+    for (int m = 0; m < B; ++m) {
+        double a = (double)((2L * i + 1) * (2L * m + 1) % (8L * B)) / (4.0 * B);
+        double term = sinpi(a) / (2.0 * m + 1.0) - comp;
+        double tsum = s + term;
+        comp = (tsum - s) - term;
+        s = tsum;
+    }
+    wts[k] = 2.0 / B * sinpi((2.0 * i + 1.0) / (4.0 * B)) * s;
+    double2 t;
+    t.x = cospi(2.0 * k / N);
+    t.y = -sinpi(2.0 * k / N);
+    tw[k] = t;
+    float2 tf;
+    tf.x = (float)t.x;
+    tf.y = (float)t.y;
+    tw32[k] = tf;
+}
+
+__global__ void k_plan_normfac(int B, double* nf)
+{
+    int M = blockIdx.x, Mp = threadIdx.x;
+    for (; Mp <= M; Mp += blockDim.x) {
+        double f = sqrt((2.0 * M + 1.0) / 2.0);
+        for (int i = 0; i < M - Mp; ++i) f *= sqrt((2.0 * M - i) / (i + 1.0));
+        nf[M * B + Mp] = f;
+    }
+}
+
+// Wigner table for one (M, M'), column-major (B - M) x 2B, with the
+// reference's base sign -1 (fn_dwt.hpp:124-125, :152-199).
+__global__ void k_wigner_table(int B, int M, int Mp, const double* cosb, const double* sinh, const double* cosh,
+                               const double* wts, const double* nf, int weighted, double* out)
+{
+    const int N = 2 * B, R = B - M;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    double b = -nf[M * B + Mp];
+    for (int e = 0; e < M - Mp; ++e) b *= sinh[k];
+    for (int e = 0; e < M + Mp; ++e) b *= cosh[k];
+    if (weighted) b *= wts[k];
+    double cb = cosb[k], dp = 0.0, d = b;
+    for (int r = 0; r < R; ++r) {
+        out[(long)k * R + r] = d;
+        if (r == R - 1) break;
+        double idx = M + r;
+        double nrm = sqrt((2.0 * idx + 3.0) / (2.0 * idx + 1.0));
+        double nom = (idx + 1.0) * (2.0 * idx + 1.0);
+        double den = 1.0 / sqrt(((idx + 1) * (idx + 1) - (double)M * M) * ((idx + 1) * (idx + 1) - (double)Mp * Mp));
+        double f1 = nrm * nom * den, f2 = 0.0, c1 = 0.0;
+        if (M + r != 0) {
+            double t1 = sqrt((2.0 * idx + 3.0) / (2.0 * idx - 1.0)) * (idx + 1.0) / idx;
+            double t2 = sqrt((idx * idx - (double)M * M) * (idx * idx - (double)Mp * Mp));
+            c1 = -t1 * t2 * den;
+            f2 = -(double)(M * Mp) / (idx * (idx + 1.0));
+        }
+        double dn = c1 * dp + d * f1 * (f2 + cb);
+        dp = d;
+        d = dn;
+    }
+}
+
+// Self-test of the FP64 MFMA lane layout the DWT kernels rely on:
+// A (16x4): lane i + 16kk holds A[i][kk]; B (4x16): lane j + 16kk holds
+// B[kk][j]; D (16x16): lane L holds D[4(L/16) + v][L%16].  With a single
+// non-zero K index ksel, A[i][ksel] = 2^i and B[ksel][j] = 3^j, every output is
+// 2^i 3^j, which names its row and column uniquely.
+__global__ void k_mfma_probe(double* out)
+{
+    const int lane = threadIdx.x;
+    for (int t = 0; t < 2; ++t) {
+        const int ksel = t == 0 ? 0 : 3;
+        double a = (lane >> 4) == ksel ? ldexp(1.0, lane & 15) : 0.0;
+        double b = 1.0;
+        for (int e = 0; e < (lane & 15); ++e) b *= 3.0;
+        if ((lane >> 4) != ksel) b = 0.0;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        for (int v = 0; v < 4; ++v) out[t * 256 + lane * 4 + v] = acc[v];
+    }
+}
+
+int mfma_selftest(hipStream_t st)
+{
+    double* d = nullptr;
+    HIP_TRY(hipMalloc(&d, 512 * sizeof(double)));
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, d);
+    double h[512];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(PFS_ERR_DEVICE, "MFMA self-test: %s", hipGetErrorString(e));
+    for (int t = 0; t < 2; ++t)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int v = 0; v < 4; ++v) {
+                double x = h[t * 256 + lane * 4 + v];
+                int i = 0, j = 0;
+                while (x > 0 && fmod(x, 2.0) == 0.0) { x /= 2.0; ++i; }
+                while (x > 1 && fmod(x, 3.0) == 0.0) { x /= 3.0; ++j; }
+                int ei = 4 * (lane >> 4) + v, ej = lane & 15;
+                if (x != 1.0 || i != ei || j != ej)
+                    return fail(PFS_ERR_UNSUPPORTED,
+                                "MFMA f64 16x16x4 layout differs from the assumed one: ksel=%d lane=%d v=%d gives "
+                                "row %d col %d (rest %g), expected row %d col %d",
+                                t ? 3 : 0, lane, v, i, j, x, ei, ej);
+            }
+    return PFS_OK;
+}
+
+struct Plan {
+    int B = 0;
+    PlanTables t{};
+    void* mem = nullptr;  // one allocation for all tables
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct pfs_context {
+    int device = 0;
+    size_t budget = 0;
+    size_t used = 0;
+    hipStream_t stream = nullptr;
+    std::map<int, Plan> plans;
+    std::mutex mu;
+    // single-signal work buffers (grown on demand)
+    void* work[2] = {nullptr, nullptr};
+    size_t work_bytes = 0;
+    // device staging for the host-buffer entry points
+    void* io_grid = nullptr;
+    size_t io_grid_bytes = 0;
+    void* io_coeff = nullptr;
+    size_t io_coeff_bytes = 0;
+    // allocations handed out by pfs_device_alloc
+    std::map<void*, size_t> user_allocs;
+};
+
+namespace {
+
+int ctx_alloc(pfs_context* c, void** p, size_t bytes)
+{
+    if (c->budget && c->used + bytes > c->budget)
+        return fail(PFS_ERR_BUDGET, "allocation of %zu bytes exceeds the context budget (%zu of %zu used)", bytes,
+                    c->used, c->budget);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(PFS_ERR_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    c->used += bytes;
+    return PFS_OK;
+}
+
+void ctx_free(pfs_context* c, void*& p, size_t bytes)
+{
+    if (p) {
+        (void)hipFree(p);
+        c->used -= bytes;
+        p = nullptr;
+    }
+}
+
+int check_bandwidth(int B)
+{
+    if (B < 1 || B > MAX_BANDWIDTH)
+        return fail(PFS_ERR_UNSUPPORTED, "bandwidth %d outside the supported range 1..%d", B, MAX_BANDWIDTH);
+    int N = 2 * B;
+    if ((N & (N - 1)) != 0 && N > 512)
+        return fail(PFS_ERR_UNSUPPORTED, "bandwidth %d: grid side %d is neither a power of two nor <= 512", B, N);
+    return PFS_OK;
+}
+
+int get_plan(pfs_context* c, int B, Plan** out)
+{
+    int rc = check_bandwidth(B);
+    if (rc) return rc;
+    auto it = c->plans.find(B);
+    if (it != c->plans.end()) {
+        *out = &it->second;
+        return PFS_OK;
+    }
+    const int N = 2 * B;
+    std::vector<int2> pairs;
+    for (int M = 0; M < B; ++M)
+        for (int Mp = 0; Mp <= M; ++Mp) pairs.push_back(make_int2(M, Mp));
+    // heaviest first: rows B - M
+    std::stable_sort(pairs.begin(), pairs.end(), [](int2 a, int2 b) { return a.x < b.x; });
+
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off_cosb = 0;
+    size_t off_sinh = off_cosb + align(N * sizeof(double));
+    size_t off_cosh = off_sinh + align(N * sizeof(double));
+    size_t off_wts = off_cosh + align(N * sizeof(double));
+    size_t off_nf = off_wts + align(N * sizeof(double));
+    size_t off_pairs = off_nf + align((size_t)B * B * sizeof(double));
+    size_t off_tw = off_pairs + align(pairs.size() * sizeof(int2));
+    size_t off_tw32 = off_tw + align(N * sizeof(double2));
+    size_t total = off_tw32 + align(N * sizeof(float2));
+
+    Plan p;
+    p.B = B;
+    rc = ctx_alloc(c, &p.mem, total);
+    if (rc) return rc;
+    p.bytes = total;
+    char* base = (char*)p.mem;
+    p.t.B = B;
+    p.t.N = N;
+    p.t.npairs = (int)pairs.size();
+    p.t.cosb = (double*)(base + off_cosb);
+    p.t.sinh = (double*)(base + off_sinh);
+    p.t.cosh = (double*)(base + off_cosh);
+    p.t.wts = (double*)(base + off_wts);
+    p.t.normfac = (double*)(base + off_nf);
+    p.t.pairs = (int2*)(base + off_pairs);
+    p.t.tw = (double2*)(base + off_tw);
+    p.t.tw32 = (float2*)(base + off_tw32);
+    hipError_t e = hipMemcpyAsync((void*)p.t.pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                  c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
+                           (double*)p.t.sinh, (double*)p.t.cosh, (double*)p.t.wts, (double2*)p.t.tw,
+                           (float2*)p.t.tw32);
+        hipLaunchKernelGGL(k_plan_normfac, dim3(B), dim3(64), 0, c->stream, B, (double*)p.t.normfac);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        ctx_free(c, p.mem, p.bytes);
+        return fail(PFS_ERR_DEVICE, "plan tables for B=%d: %s", B, hipGetErrorString(e));
+    }
+    auto ins = c->plans.emplace(B, p);
+    *out = &ins.first->second;
+    return PFS_OK;
+}
+
+int ensure_work(pfs_context* c, size_t bytes)
+{
+    if (c->work_bytes >= bytes) return PFS_OK;
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, c->work[0], c->work_bytes);
+    ctx_free(c, c->work[1], c->work_bytes);
+    c->work_bytes = 0;
+    int rc = ctx_alloc(c, &c->work[0], bytes);
+    if (rc) return rc;
+    rc = ctx_alloc(c, &c->work[1], bytes);
+    if (rc) {
+        ctx_free(c, c->work[0], bytes);
+        return rc;
+    }
+    c->work_bytes = bytes;
+    return PFS_OK;
+}
+
+int ensure_io(pfs_context* c, size_t gbytes, size_t cbytes)
+{
+    if (c->io_grid_bytes < gbytes) {
+        ctx_free(c, c->io_grid, c->io_grid_bytes);
+        c->io_grid_bytes = 0;
+        int rc = ctx_alloc(c, &c->io_grid, gbytes);
+        if (rc) return rc;
+        c->io_grid_bytes = gbytes;
+    }
+    if (c->io_coeff_bytes < cbytes) {
+        ctx_free(c, c->io_coeff, c->io_coeff_bytes);
+        c->io_coeff_bytes = 0;
+        int rc = ctx_alloc(c, &c->io_coeff, cbytes);
+        if (rc) return rc;
+        c->io_coeff_bytes = cbytes;
+    }
+    return PFS_OK;
+}
+
+inline size_t grid_bytes(int B) { size_t N = 2 * (size_t)B; return N * N * N * sizeof(double2); }
+inline size_t coeff_bytes(int B) { return (size_t)coeff_offset(B) * sizeof(double2); }
+
+// Forward row pass: x[k][c][r] -> W[k][p][c]  (lines u = k, v = c).
+LineMap map_rows_fwd(int N)
+{
+    long n = N, n2 = n * n;
+    LineMap m{n2, n, 1, n2, 1, n, 0, 0, N, N};
+    return m;
+}
+// Forward column pass: W[k][p][c] -> G[p][q][k]  (lines u = p, v = k).
+LineMap map_cols_fwd(int N)
+{
+    long n = N, n2 = n * n;
+    LineMap m{n, n2, 1, n2, 1, n, 0, 0, N, N};
+    return m;
+}
+// Inverse column pass: G[p][q][k] -> W[k][p][c]  (lines u = p, v = k).
+LineMap map_cols_inv(int N)
+{
+    long n = N, n2 = n * n;
+    LineMap m{n2, 1, n, n, n2, 1, 0, 0, N, N};
+    return m;
+}
+// Inverse row pass: W[k][p][c] -> x[k][c][r]  (lines u = k, v = c).
+LineMap map_rows_inv(int N)
+{
+    long n = N, n2 = n * n;
+    LineMap m{n2, 1, n, n2, n, 1, 0, 0, N, N};
+    return m;
+}
+// In-place 2D DFT, second pass: W[k][p][c] -> x[k][q][p]  (lines u = k, v = p).
+LineMap map_cols_seam(int N)
+{
+    long n = N, n2 = n * n;
+    LineMap m{n2, n, 1, n2, 1, n, 0, 0, N, N};
+    return m;
+}
+
+hipStream_t pick(pfs_context* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+int fft_fwd(pfs_context* c, const Plan& p, const double2* grid, double2* W, double2* G, hipStream_t st)
+{
+    const int N = p.t.N;
+    HIP_TRY(launch_fft_lines_f64(grid, W, map_rows_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    HIP_TRY(launch_fft_lines_f64(W, G, map_cols_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    return PFS_OK;
+}
+
+int fft_inv(pfs_context* c, const Plan& p, const double2* G, double2* W, double2* grid, hipStream_t st)
+{
+    const int N = p.t.N;
+    const int B = p.B;
+    HIP_TRY(launch_fft_lines_f64(G, W, map_cols_inv(N), N, +1, 1.0, p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st));
+    HIP_TRY(launch_fft_lines_f64(W, grid, map_rows_inv(N), N, +1, 1. / (4. * B * B), p.t.tw, LINES_TILE_IN_ROW_OUT,
+                                 1, st));
+    return PFS_OK;
+}
+
+// pi / (B * 2B) and (B * 2B) / pi in double (src/fn_dsoft.cpp:119, src/fn_idsoft.cpp:109)
+inline double fwd_norm(int B) { return 3.14159265358979323846 / (B * (2 * B)); }
+inline double inv_norm(int B) { return (B * (2 * B)) / 3.14159265358979323846; }
+
+int dsoft_dev(pfs_context* c, int B, const double2* grid, double2* coeffs, hipStream_t st)
+{
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    rc = ensure_work(c, grid_bytes(B));
+    if (rc) return rc;
+    double2* W = (double2*)c->work[0];
+    double2* G = (double2*)c->work[1];
+    rc = fft_fwd(c, *p, grid, W, G, st);
+    if (rc) return rc;
+    HIP_TRY(launch_dwt_fwd_f64(p->t, G, coeffs, fwd_norm(B), st));
+    return PFS_OK;
+}
+
+int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipStream_t st)
+{
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    rc = ensure_work(c, grid_bytes(B));
+    if (rc) return rc;
+    double2* W = (double2*)c->work[0];
+    double2* G = (double2*)c->work[1];
+    HIP_TRY(launch_nyquist_gather_f64(grid, G, B, st));
+    HIP_TRY(launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st));
+    return fft_inv(c, *p, G, W, grid, st);
+}
+
+int dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double scale, hipStream_t st)
+{
+    if (!c || !arr_dev || n < 2 || (n & 1) || lays < 1 || (sign != 1 && sign != -1))
+        return fail(PFS_ERR_ARG, "pfs_layer_wise_dft2_dev: bad arguments (n must be even and >= 2)");
+    int B = n / 2;
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    size_t bytes = (size_t)n * n * lays * sizeof(double2);
+    rc = ensure_work(c, bytes);
+    if (rc) return rc;
+    double2* a = (double2*)arr_dev;
+    double2* W = (double2*)c->work[0];
+    LineMap m1 = map_rows_fwd(n);
+    m1.nu = lays;
+    LineMap m2 = map_cols_seam(n);
+    m2.nu = lays;
+    HIP_TRY(launch_fft_lines_f64(a, W, m1, n, sign, 1.0, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    HIP_TRY(launch_fft_lines_f64(W, a, m2, n, sign, scale, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    return PFS_OK;
+}
+
+std::mutex g_seam_mu;
+pfs_context* g_seam_ctx = nullptr;
+
+int seam_dft2(int cols, int rows, int lays, double* arr, int sign)
+{
+    g_last_error.clear();  // the seam returns void: callers read pfs_last_error()
+    if (!arr || cols < 1 || rows < 1 || lays < 0) return fail(PFS_ERR_ARG, "layer_wise DFT2: bad arguments");
+    if (cols != rows) return fail(PFS_ERR_UNSUPPORTED, "layer_wise DFT2: only square slices (cols == rows) are supported");
+    if (lays == 0) return PFS_OK;
+    std::lock_guard<std::mutex> g(g_seam_mu);
+    if (!g_seam_ctx) {
+        int rc = pfs_ctx_create(0, 0, &g_seam_ctx);
+        if (rc) return rc;
+    }
+    pfs_context* c = g_seam_ctx;
+    std::lock_guard<std::mutex> g2(c->mu);
+    size_t bytes = (size_t)cols * rows * lays * sizeof(double2);
+    int rc = ensure_io(c, bytes, 0);
+    if (rc) return rc;
+    rc = ensure_work(c, bytes);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->io_grid, arr, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipSetDevice(c->device));
+    rc = dft2_dev(c, cols, lays, c->io_grid, sign, 1.0, c->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(arr, c->io_grid, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFS_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* pfs_last_error(void) { return g_last_error.c_str(); }
+
+const char* pfs_version(void) { return "pfsoft_amd 0.1.0 (gfx950)"; }
+
+int pfs_ctx_create(int device, size_t hbm_budget_bytes, pfs_context** out)
+{
+    if (!out) return fail(PFS_ERR_ARG, "pfs_ctx_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(PFS_ERR_ARG, "pfs_ctx_create: device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    pfs_context* c = new pfs_context;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PFS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    int rc = mfma_selftest(c->stream);
+    if (rc) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return PFS_OK;
+}
+
+int pfs_ctx_destroy(pfs_context* c)
+{
+    if (!c) return PFS_OK;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto& kv : c->plans) ctx_free(c, kv.second.mem, kv.second.bytes);
+        c->plans.clear();
+        ctx_free(c, c->work[0], c->work_bytes);
+        ctx_free(c, c->work[1], c->work_bytes);
+        ctx_free(c, c->io_grid, c->io_grid_bytes);
+        ctx_free(c, c->io_coeff, c->io_coeff_bytes);
+        for (auto& kv : c->user_allocs) {
+            void* ptr = kv.first;
+            ctx_free(c, ptr, kv.second);
+        }
+        c->user_allocs.clear();
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return PFS_OK;
+}
+
+int pfs_plan_warm(pfs_context* c, int B)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    return get_plan(c, B, &p);
+}
+
+long pfs_coeff_count(int B) { return B < 0 ? 0 : coeff_offset(B); }
+
+long pfs_coeff_index(int l, int M, int Mp)
+{
+    if (l < 0 || M > l || M < -l || Mp > l || Mp < -l) return -1;
+    return coeff_index(l, M, Mp);
+}
+
+int pfs_dsoft_f64(pfs_context* c, int B, const double* grid, double* coeffs)
+{
+    if (!c || !grid || !coeffs) return fail(PFS_ERR_ARG, "pfs_dsoft_f64: null argument");
+    int rc = check_bandwidth(B);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    rc = ensure_io(c, grid_bytes(B), coeff_bytes(B));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->io_grid, grid, grid_bytes(B), hipMemcpyHostToDevice, c->stream));
+    rc = dsoft_dev(c, B, (const double2*)c->io_grid, (double2*)c->io_coeff, c->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(coeffs, c->io_coeff, coeff_bytes(B), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFS_OK;
+}
+
+int pfs_idsoft_f64(pfs_context* c, int B, const double* coeffs, double* grid)
+{
+    if (!c || !grid || !coeffs) return fail(PFS_ERR_ARG, "pfs_idsoft_f64: null argument");
+    int rc = check_bandwidth(B);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    rc = ensure_io(c, grid_bytes(B), coeff_bytes(B));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->io_coeff, coeffs, coeff_bytes(B), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->io_grid, grid, grid_bytes(B), hipMemcpyHostToDevice, c->stream));
+    rc = idsoft_dev(c, B, (const double2*)c->io_coeff, (double2*)c->io_grid, c->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(grid, c->io_grid, grid_bytes(B), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFS_OK;
+}
+
+int pfs_dsoft_f64_dev(pfs_context* c, int B, const void* grid_dev, void* coeffs_dev, void* stream)
+{
+    if (!c || !grid_dev || !coeffs_dev) return fail(PFS_ERR_ARG, "pfs_dsoft_f64_dev: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return dsoft_dev(c, B, (const double2*)grid_dev, (double2*)coeffs_dev, pick(c, stream));
+}
+
+int pfs_idsoft_f64_dev(pfs_context* c, int B, const void* coeffs_dev, void* grid_dev, void* stream)
+{
+    if (!c || !grid_dev || !coeffs_dev) return fail(PFS_ERR_ARG, "pfs_idsoft_f64_dev: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return idsoft_dev(c, B, (const double2*)coeffs_dev, (double2*)grid_dev, pick(c, stream));
+}
+
+void pfs_layer_wise_DFT2_grid3D(int cols, int rows, int lays, double* arr, int threads)
+{
+    (void)threads;
+    int rc = seam_dft2(cols, rows, lays, arr, -1);
+    if (rc) fprintf(stderr, "** [pfsoft_amd error] layer_wise_DFT2_grid3D: %s **\n", pfs_last_error());
+}
+
+void pfs_layer_wise_IDFT2_grid3D(int cols, int rows, int lays, double* arr, int threads)
+{
+    (void)threads;
+    int rc = seam_dft2(cols, rows, lays, arr, +1);
+    if (rc) fprintf(stderr, "** [pfsoft_amd error] layer_wise_IDFT2_grid3D: %s **\n", pfs_last_error());
+}
+
+int pfs_layer_wise_dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double scale, void* stream)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return dft2_dev(c, n, lays, arr_dev, sign, scale, pick(c, stream));
+}
+
+int pfs_quadrature_weights(pfs_context* c, int B, double* out)
+{
+    if (!c || !out) return fail(PFS_ERR_ARG, "pfs_quadrature_weights: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, p->t.wts, 2 * (size_t)B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFS_OK;
+}
+
+int pfs_wigner_d_matrix(pfs_context* c, int B, int M, int Mp, int weighted, double* out)
+{
+    if (!c || !out) return fail(PFS_ERR_ARG, "pfs_wigner_d_matrix: null argument");
+    if (!(M >= Mp && Mp >= 0 && M < B)) return fail(PFS_ERR_ARG, "pfs_wigner_d_matrix: need B > M >= M' >= 0");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    size_t bytes = (size_t)(B - M) * 2 * B * sizeof(double);
+    rc = ensure_io(c, bytes, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_wigner_table, dim3((2 * B + 63) / 64), dim3(64), 0, c->stream, B, M, Mp, p->t.cosb,
+                       p->t.sinh, p->t.cosh, p->t.wts, p->t.normfac, weighted, (double*)c->io_grid);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, c->io_grid, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFS_OK;
+}
+
+int pfs_stage_fft_fwd_dev(pfs_context* c, int B, const void* grid_dev, void* pencils_dev, void* stream)
+{
+    if (!c || !grid_dev || !pencils_dev) return fail(PFS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    rc = ensure_work(c, grid_bytes(B));
+    if (rc) return rc;
+    return fft_fwd(c, *p, (const double2*)grid_dev, (double2*)c->work[0], (double2*)pencils_dev, pick(c, stream));
+}
+
+int pfs_stage_dwt_fwd_dev(pfs_context* c, int B, const void* pencils_dev, void* coeffs_dev, void* stream)
+{
+    if (!c || !pencils_dev || !coeffs_dev) return fail(PFS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    HIP_TRY(launch_dwt_fwd_f64(p->t, (const double2*)pencils_dev, (double2*)coeffs_dev, fwd_norm(B), pick(c, stream)));
+    return PFS_OK;
+}
+
+int pfs_stage_dwt_inv_dev(pfs_context* c, int B, const void* coeffs_dev, void* pencils_dev, void* stream)
+{
+    if (!c || !pencils_dev || !coeffs_dev) return fail(PFS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    HIP_TRY(launch_dwt_inv_f64(p->t, (const double2*)coeffs_dev, (double2*)pencils_dev, inv_norm(B), pick(c, stream)));
+    return PFS_OK;
+}
+
+int pfs_stage_fft_inv_dev(pfs_context* c, int B, const void* pencils_dev, void* grid_dev, void* stream)
+{
+    if (!c || !pencils_dev || !grid_dev) return fail(PFS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    rc = ensure_work(c, grid_bytes(B));
+    if (rc) return rc;
+    return fft_inv(c, *p, (const double2*)pencils_dev, (double2*)c->work[0], (double2*)grid_dev, pick(c, stream));
+}
+
+int pfs_device_alloc(pfs_context* c, size_t bytes, void** out)
+{
+    if (!c || !out) return fail(PFS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    int rc = ctx_alloc(c, out, bytes);
+    if (rc == PFS_OK) c->user_allocs[*out] = bytes;
+    return rc;
+}
+
+int pfs_device_free(pfs_context* c, void* ptr)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null argument");
+    if (!ptr) return PFS_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    auto it = c->user_allocs.find(ptr);
+    if (it == c->user_allocs.end()) return fail(PFS_ERR_ARG, "pfs_device_free: pointer not allocated by this context");
+    size_t bytes = it->second;
+    c->user_allocs.erase(it);
+    ctx_free(c, ptr, bytes);
+    return PFS_OK;
+}
+
+int pfs_memcpy_h2d(pfs_context* c, void* dst, const void* src, size_t bytes)
+{
+    if (!c || !dst || !src) return fail(PFS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PFS_OK;
+}
+
+int pfs_memcpy_d2h(pfs_context* c, void* dst, const void* src, size_t bytes)
+{
+    if (!c || !dst || !src) return fail(PFS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PFS_OK;
+}
+
+int pfs_synchronize(pfs_context* c, void* stream)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(pick(c, stream)));
+    return PFS_OK;
+}
+
+}  // extern "C"

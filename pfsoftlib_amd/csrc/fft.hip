@@ -1,0 +1,352 @@
+// Slice FFT stage: batched length-N line DFTs for the layer-wise 2D FFT.
+//
+// A 2D FFT of every beta-slice is done as two passes of line transforms, each
+// one HBM read + one HBM write of the grid.  A workgroup owns LINES lines:
+// it loads them into LDS (coalesced: either whole rows, or LINES adjacent
+// elements per n), runs a Stockham autosort FFT (radix 8/4/2 passes, one
+// barrier-separated in-place LDS exchange per pass) and stores the lines in
+// the other orientation, which is how the slice transposes and the
+// beta-fastest pencil layout are produced without extra passes.
+// Non-power-of-two N (any even N up to 512) uses a direct O(N^2) DFT per line
+// with the same load/store paths.
+//
+// Contract restated from src/fftw_wrapper.cpp:47-172: unnormalised,
+// FFTW_FORWARD = exp(-2 pi i jk/N) (sign -1), FFTW_BACKWARD sign +1.
+#include "pfs_internal.h"
+
+namespace pfs {
+
+template <typename T> struct CT;
+template <> struct CT<double> { typedef double2 C; };
+template <> struct CT<float> { typedef float2 C; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+// a * (SIGN * i)
+template <int SIGN, typename C> __device__ __forceinline__ C cmul_i(C a)
+{
+    C r;
+    if (SIGN > 0) { r.x = -a.y; r.y = a.x; }
+    else          { r.x = a.y;  r.y = -a.x; }
+    return r;
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft2(C* v)
+{
+    C t = v[0];
+    v[0] = cadd(t, v[1]);
+    v[1] = csub(t, v[1]);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft4(C& x0, C& x1, C& x2, C& x3)
+{
+    C a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = cmul_i<SIGN>(csub(x1, x3));
+    x0 = cadd(a, c);
+    x2 = csub(a, c);
+    x1 = cadd(b, d);
+    x3 = csub(b, d);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft8(C* v)
+{
+    typedef decltype(v[0].x) R;
+    const R h = (R)0.70710678118654752440084436210484904;
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<SIGN>(e0, e1, e2, e3);
+    dft4<SIGN>(o0, o1, o2, o3);
+    // o_k *= w^k, w = exp(SIGN 2 pi i / 8)
+    C t1, t3;
+    t1.x = (o1.x - SIGN * o1.y) * h;
+    t1.y = (o1.y + SIGN * o1.x) * h;
+    C t2 = cmul_i<SIGN>(o2);
+    t3.x = (-o3.x - SIGN * o3.y) * h;
+    t3.y = (-o3.y + SIGN * o3.x) * h;
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1); v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2); v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3); v[7] = csub(e3, t3);
+}
+
+template <int R, int SIGN, typename C> __device__ __forceinline__ void dftR(C* v)
+{
+    if (R == 2) dft2<SIGN>(v);
+    else if (R == 4) dft4<SIGN>(v[0], v[1], v[2], v[3]);
+    else dft8<SIGN>(v);
+}
+
+__host__ __device__ constexpr int n_passes(int logn) { return logn / 3 + (logn % 3 ? 1 : 0); }
+__host__ __device__ constexpr int radix_at(int logn, int p) { return p < logn / 3 ? 8 : (logn % 3 == 2 ? 4 : 2); }
+
+// One in-place Stockham pass over LINES lines of length N held in LDS.
+template <int N, int LINES, int R, int SIGN, typename C>
+__device__ __forceinline__ void stockham_pass(C* lds, int ls, int ns, const C* __restrict__ tw)
+{
+    constexpr int NB = N / R;                 // butterflies per line
+    constexpr int TOT = NB * LINES;
+    constexpr int PER = (TOT + 255) / 256;
+    C v[PER][R];
+    int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int b = tid + 256 * q;
+        if (TOT % 256 == 0 || b < TOT) {
+            int line = b / NB, j = b % NB;
+            const C* src = lds + line * ls;
+            int k = j % ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = src[j + r * NB];
+            if (ns > 1) {
+                int step = N / (ns * R);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C w = tw[r * k * step];
+                    if (SIGN > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dftR<R, SIGN>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int b = tid + 256 * q;
+        if (TOT % 256 == 0 || b < TOT) {
+            int line = b / NB, j = b % NB;
+            C* dst = lds + line * ls;
+            int k = j % ns;
+            int base = (j / ns) * ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[base + r * ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int LOGN, int LINES, int SIGN, typename C>
+__device__ __forceinline__ void fft_in_lds(C* lds, int ls, const C* __restrict__ tw)
+{
+    constexpr int N = 1 << LOGN;
+    int ns = 1;
+#pragma unroll
+    for (int p = 0; p < n_passes(LOGN); ++p) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int R = radix_at(LOGN, p);
+        if (R == 8) stockham_pass<N, LINES, (LOGN >= 3 ? 8 : 2), SIGN>(lds, ls, ns, tw);
+        else if (R == 4) stockham_pass<N, LINES, (LOGN >= 2 ? 4 : 2), SIGN>(lds, ls, ns, tw);
+        else stockham_pass<N, LINES, 2, SIGN>(lds, ls, ns, tw);
+        ns *= R;
+    }
+}
+
+// Load LINES lines of the tile into LDS.
+template <int KIND, typename C>
+__device__ __forceinline__ void load_tile(C* lds, int ls, const C* __restrict__ in, long base, long sv, long sn,
+                                          int N, int lines)
+{
+    int tid = threadIdx.x;
+    int tot = lines * N;
+    if (KIND == LINES_ROW_IN_TILE_OUT) {
+        for (int e = tid; e < tot; e += 256) {
+            int line = e / N, n = e - line * N;
+            lds[line * ls + n] = in[base + line * sv + (long)n * sn];
+        }
+    } else {
+        for (int e = tid; e < tot; e += 256) {
+            int n = e / lines, line = e - n * lines;
+            lds[line * ls + n] = in[base + line * sv + (long)n * sn];
+        }
+    }
+}
+
+template <int KIND, typename C, typename T>
+__device__ __forceinline__ void store_tile(const C* lds, int ls, C* __restrict__ out, long base, long sv, long sn,
+                                           int N, int lines, T scale)
+{
+    int tid = threadIdx.x;
+    int tot = lines * N;
+    if (KIND == LINES_ROW_IN_TILE_OUT) {
+        for (int e = tid; e < tot; e += 256) {
+            int n = e / lines, line = e - n * lines;
+            C v = lds[line * ls + n];
+            v.x *= scale;
+            v.y *= scale;
+            out[base + line * sv + (long)n * sn] = v;
+        }
+    } else {
+        for (int e = tid; e < tot; e += 256) {
+            int line = e / N, n = e - line * N;
+            C v = lds[line * ls + n];
+            v.x *= scale;
+            v.y *= scale;
+            out[base + line * sv + (long)n * sn] = v;
+        }
+    }
+}
+
+template <typename T, int LOGN, int LINES, int KIND, int SIGN>
+__global__ __launch_bounds__(256) void k_fft_lines(const typename CT<T>::C* __restrict__ in,
+                                                   typename CT<T>::C* __restrict__ out, LineMap m,
+                                                   const typename CT<T>::C* __restrict__ tw, T scale)
+{
+    typedef typename CT<T>::C C;
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int ls = N + 1;
+    const long u = blockIdx.y, v0 = (long)blockIdx.x * LINES, sig = blockIdx.z;
+    const long ib = sig * m.in_sig + u * m.in_su + v0 * m.in_sv;
+    const long ob = sig * m.out_sig + u * m.out_su + v0 * m.out_sv;
+    load_tile<KIND>(lds, ls, in, ib, m.in_sv, m.in_sn, N, LINES);
+    __syncthreads();
+    fft_in_lds<LOGN, LINES, SIGN>(lds, ls, tw);
+    store_tile<KIND>(lds, ls, out, ob, m.out_sv, m.out_sn, N, LINES, scale);
+}
+
+// Direct DFT for even N that is not a power of two.
+template <typename T, int KIND, int SIGN>
+__global__ __launch_bounds__(256) void k_dft_lines(const typename CT<T>::C* __restrict__ in,
+                                                   typename CT<T>::C* __restrict__ out, LineMap m,
+                                                   const typename CT<T>::C* __restrict__ tw, T scale, int N,
+                                                   int lines)
+{
+    typedef typename CT<T>::C C;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* a = reinterpret_cast<C*>(smem_raw);
+    const int ls = N + 1;
+    C* b = a + lines * ls;
+    const long u = blockIdx.y, v0 = (long)blockIdx.x * lines, sig = blockIdx.z;
+    const long ib = sig * m.in_sig + u * m.in_su + v0 * m.in_sv;
+    const long ob = sig * m.out_sig + u * m.out_su + v0 * m.out_sv;
+    load_tile<KIND>(a, ls, in, ib, m.in_sv, m.in_sn, N, lines);
+    __syncthreads();
+    for (int e = threadIdx.x; e < lines * N; e += 256) {
+        int line = e / N, k = e - line * N;
+        const C* x = a + line * ls;
+        T sr = 0, si = 0;
+        int idx = 0;
+        for (int j = 0; j < N; ++j) {
+            C w = tw[idx];
+            if (SIGN > 0) w.y = -w.y;
+            sr += x[j].x * w.x - x[j].y * w.y;
+            si += x[j].x * w.y + x[j].y * w.x;
+            idx += k;
+            if (idx >= N) idx -= N;
+        }
+        C r;
+        r.x = sr;
+        r.y = si;
+        b[line * ls + k] = r;
+    }
+    __syncthreads();
+    store_tile<KIND>(b, ls, out, ob, m.out_sv, m.out_sn, N, lines, scale);
+}
+
+template <typename T, int LOGN, int KIND, int SIGN>
+static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m,
+                              const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;
+    size_t smem = (size_t)LINES * (N + 1) * sizeof(typename CT<T>::C);
+    if (smem > lds_limit()) return hipErrorInvalidConfiguration;
+    dim3 grid(m.nv / LINES, m.nu, nsig);
+    hipLaunchKernelGGL((k_fft_lines<T, LOGN, LINES, KIND, SIGN>), grid, dim3(256), smem, st, in, out, m, tw, scale);
+    return hipGetLastError();
+}
+
+template <typename T, int KIND, int SIGN>
+static hipError_t dispatch_pow2(int logn, const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m,
+                                const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st)
+{
+    switch (logn) {
+    case 1: return launch_pow2<T, 1, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 2: return launch_pow2<T, 2, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 3: return launch_pow2<T, 3, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 4: return launch_pow2<T, 4, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 5: return launch_pow2<T, 5, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 6: return launch_pow2<T, 6, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 7: return launch_pow2<T, 7, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 8: return launch_pow2<T, 8, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 9: return launch_pow2<T, 9, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 10: return launch_pow2<T, 10, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m, int N,
+                               int sign, T scale, const typename CT<T>::C* tw, int kind, int nsig, hipStream_t st)
+{
+    if (N < 2 || (N & 1) || m.nv != N || nsig < 1) return hipErrorInvalidValue;
+    int logn = 0;
+    while ((1 << logn) < N) ++logn;
+    if ((1 << logn) == N) {
+        if (logn > 10) return hipErrorInvalidValue;
+        if (kind == LINES_ROW_IN_TILE_OUT)
+            return sign < 0 ? dispatch_pow2<T, LINES_ROW_IN_TILE_OUT, -1>(logn, in, out, m, tw, scale, nsig, st)
+                            : dispatch_pow2<T, LINES_ROW_IN_TILE_OUT, 1>(logn, in, out, m, tw, scale, nsig, st);
+        return sign < 0 ? dispatch_pow2<T, LINES_TILE_IN_ROW_OUT, -1>(logn, in, out, m, tw, scale, nsig, st)
+                        : dispatch_pow2<T, LINES_TILE_IN_ROW_OUT, 1>(logn, in, out, m, tw, scale, nsig, st);
+    }
+    if (N > 512) return hipErrorInvalidValue;
+    int lines = 0;
+    for (int cand : {8, 4, 2})
+        if (N % cand == 0 && 2 * (size_t)cand * (N + 1) * sizeof(typename CT<T>::C) <= 65536) {
+            lines = cand;
+            break;
+        }
+    if (!lines) return hipErrorInvalidValue;
+    size_t smem = 2 * (size_t)lines * (N + 1) * sizeof(typename CT<T>::C);
+    dim3 grid(m.nv / lines, m.nu, nsig);
+    if (kind == LINES_ROW_IN_TILE_OUT) {
+        if (sign < 0)
+            hipLaunchKernelGGL((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+        else
+            hipLaunchKernelGGL((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+    } else {
+        if (sign < 0)
+            hipLaunchKernelGGL((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+        else
+            hipLaunchKernelGGL((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+    }
+    return hipGetLastError();
+}
+
+size_t lds_limit()
+{
+    static int cached[64];
+    static bool init[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+    if (!init[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
+        cached[dev] = v;
+        init[dev] = true;
+    }
+    return (size_t)cached[dev];
+}
+
+hipError_t launch_fft_lines_f64(const double2* in, double2* out, const LineMap& m, int N, int sign, double scale,
+                                const double2* tw, int kind, int nsig, hipStream_t st)
+{
+    return launch_lines<double>(in, out, m, N, sign, scale, tw, kind, nsig, st);
+}
+
+hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m, int N, int sign, float scale,
+                                const float2* tw, int kind, int nsig, hipStream_t st)
+{
+    return launch_lines<float>(in, out, m, N, sign, scale, tw, kind, nsig, st);
+}
+
+}  // namespace pfs

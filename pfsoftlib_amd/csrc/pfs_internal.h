@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP translation units of libpfsoft_amd.
+//
+// Layouts in HBM (N = 2B, complex values interleaved re/im):
+//   grid    x[k][c][r]   element (row r, col c, beta-slice k) at N*N*k + N*c + r
+//                        (the reference's grid3D index, grid3D_cx.hpp:293-303)
+//   work W  W[k][p][c]   slice-transposed after the first FFT pass
+//   pencils G[p][q][k]   F(p, q, k) at (p*N + q)*N + k: the 2D spectrum of every
+//                        beta-slice with the beta index fastest, so the DWT reads
+//                        each (p, q) beta-vector as one contiguous 16*N byte run
+//   coeffs  packed       degree-l block at offset(l) = (4l^3 - l)/3, element
+//                        (l, M, M') at offset(l) + iM + (2l+1)*iM', iX = X>=0 ? X : 2l+1+X
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace pfs {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline long coeff_offset(int l) { return ((long)4 * l * l * l - l) / 3; }
+
+__host__ __device__ inline long coeff_index(int l, int M, int Mp)
+{
+    long n = 2 * l + 1;
+    long iM = M >= 0 ? M : n + M;
+    long iMp = Mp >= 0 ? Mp : n + Mp;
+    return coeff_offset(l) + iM + n * iMp;
+}
+
+// Device-resident tables for one bandwidth (all FP64 unless noted).
+struct PlanTables {
+    int B, N;
+    int npairs;
+    const double* cosb;    // cos(beta_k)
+    const double* sinh;    // sin(beta_k / 2)
+    const double* cosh;    // cos(beta_k / 2)
+    const double* wts;     // quadrature weights w_B(k)
+    const double* normfac; // sqrt((2M+1)/2) * sqrt(C(2M, M-M')), [M*B + M'], M >= M' >= 0
+    const int2* pairs;     // canonical (M, M'), M >= M' >= 0, heaviest first
+    const double2* tw;     // exp(-2 pi i j / N), j < N
+    const float2* tw32;    // same, rounded to FP32
+};
+
+// Largest dynamic LDS allocation a workgroup may request on the current device.
+size_t lds_limit();
+
+// ---- FFT stage -----------------------------------------------------------
+// A batch of length-N line transforms over a 3D array.  Line (u, v) element n
+// sits at u*su + v*sv + n*sn (complex elements) on each side; u runs over
+// blockIdx.y, v over the tiles of blockIdx.x, signals over blockIdx.z.
+struct LineMap {
+    long in_su, in_sv, in_sn;
+    long out_su, out_sv, out_sn;
+    long in_sig, out_sig;  // per-signal strides
+    int nu, nv;            // extents of u and v
+};
+
+enum LineKind { LINES_ROW_IN_TILE_OUT = 0, LINES_TILE_IN_ROW_OUT = 1 };
+
+// sign: -1 forward (FFTW_FORWARD), +1 backward.  scale multiplies the result.
+hipError_t launch_fft_lines_f64(const double2* in, double2* out, const LineMap& m, int N, int sign,
+                                double scale, const double2* tw, int kind, int nsig, hipStream_t st);
+hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m, int N, int sign,
+                                float scale, const float2* tw, int kind, int nsig, hipStream_t st);
+
+// ---- DWT stage -----------------------------------------------------------
+// Forward: G pencils (FP64) -> packed coefficients, scaled by pi/(2B^2).
+hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm,
+                              hipStream_t st);
+// Inverse: packed coefficients -> G pencils (entries with p == B or q == B untouched).
+hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm,
+                              hipStream_t st);
+
+// Copy the grid entries the inverse DWT never writes (row == B or col == B)
+// into the pencil buffer, so the inverse FFT reads the caller's values there.
+hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st);
+
+}  // namespace pfs

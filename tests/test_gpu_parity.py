@@ -1,0 +1,204 @@
+"""GPU parity: the HIP path against the CPU oracle, through the C ABI.
+
+Bar (FP64 plain path): relative L2 <= 1e-11 against the oracle on the same
+seeded inputs, and the reference's own round-trip check (every |re|, |im|
+difference <= 1e-11, examples/example.cpp:45-74).  At the largest size the
+oracle is too slow for a test, so B=256 is checked through size-independent
+properties (round trip, linearity).
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    return _t
+
+
+@pytest.fixture(scope="module")
+def ctx(torch):
+    import pfsoftlib_amd as P
+    c = P.Context(0)
+    yield c
+    c.close()
+
+
+def test_context_and_mfma_layout_selftest(ctx, torch):
+    # Context creation runs the FP64 MFMA lane-layout self-test on the device
+    assert ctx.handle
+    p = torch.cuda.get_device_properties(0)
+    print("device:", p.name, "CUs:", p.multi_processor_count, "LDS/block:",
+          getattr(p, "shared_memory_per_block", None), "L2:", getattr(p, "L2_cache_size", None))
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 8, 64, 128, 256])
+def test_quadrature_weights_on_device(ctx, B):
+    w = ctx.quadrature_weights(B)
+    assert np.abs(w - O.quadrature_weights(B)).max() < 1e-15
+
+
+@pytest.mark.parametrize("B,M,Mp", [(8, 0, 0), (8, 5, 2), (64, 0, 0), (64, 31, 7), (64, 63, 63),
+                                    (256, 0, 0), (256, 100, 37), (256, 200, 0)])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_wigner_table_on_device(ctx, B, M, Mp, weighted):
+    d = ctx.wigner_d_matrix(B, M, Mp, weighted)
+    ref = O.wigner_d(B, M, Mp, weighted)
+    assert np.linalg.norm(d - ref) / np.linalg.norm(ref) < 1e-12
+
+
+@pytest.mark.parametrize("n,lays", [(2, 2), (4, 3), (6, 4), (8, 8), (12, 5), (16, 16), (20, 3), (64, 8),
+                                    (256, 4), (512, 2)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_layer_wise_dft2_seam(n, lays, inverse):
+    import pfsoftlib_amd as P
+    rng = np.random.default_rng(n * 7 + lays)
+    a = rng.standard_normal((lays, n, n)) + 1j * rng.standard_normal((lays, n, n))
+    got = P.layer_wise_DFT2_grid3D(a.copy(), inverse=inverse)
+    ref = O.fft2_layers(a, +1 if inverse else -1)
+    assert rel(got, ref) < 1e-14
+
+
+@pytest.mark.parametrize("B", [4, 16, 128])
+def test_fft_stage_pencil_layout(ctx, torch, B):
+    N = 2 * B
+    rng = np.random.default_rng(B)
+    g = rng.standard_normal(N ** 3) + 1j * rng.standard_normal(N ** 3)
+    gd = torch.from_numpy(g).cuda()
+    pen = torch.empty_like(gd)
+    ctx.stage_fft_fwd(B, gd, pen)
+    ctx.synchronize()
+    F = O.fft2_layers(g.reshape(N, N, N), -1)           # [k][q][p]
+    ref = np.transpose(F, (2, 1, 0)).reshape(-1)         # [p][q][k]
+    assert rel(pen.cpu().numpy(), ref) < 1e-14
+    back = torch.empty_like(gd)
+    ctx.stage_fft_inv(B, pen, back)
+    ctx.synchronize()
+    assert rel(back.cpu().numpy(), g) < 1e-13
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 19, 20, 32, 33, 64])
+def test_idsoft_matches_oracle(ctx, B):
+    c = O.seeded_coeffs(B, 1000 + B)
+    g = ctx.idsoft(B, c)
+    assert rel(g, O.idsoft(B, c)) < TOL
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 19, 20, 32, 33, 64])
+def test_dsoft_matches_oracle_on_arbitrary_grid(ctx, B):
+    N = 2 * B
+    rng = np.random.default_rng(B)
+    g = rng.uniform(-1, 1, N ** 3) + 1j * rng.uniform(-1, 1, N ** 3)
+    assert rel(ctx.dsoft(B, g), O.dsoft(B, g)) < TOL
+
+
+@pytest.fixture(scope="module")
+def b128(ctx):
+    B = 128
+    c = O.seeded_coeffs(B, 2026)
+    g_ref = O.idsoft(B, c)
+    return B, c, g_ref, O.dsoft(B, g_ref)
+
+
+def test_idsoft_b128_matches_oracle(ctx, b128):
+    B, c, g_ref, _ = b128
+    assert rel(ctx.idsoft(B, c), g_ref) < TOL
+
+
+def test_dsoft_b128_matches_oracle(ctx, b128):
+    B, c, g_ref, c_ref = b128
+    got = ctx.dsoft(B, g_ref)
+    assert rel(got, c_ref) < TOL
+    # the reference's own pass criterion on the round trip (example.cpp:45-54)
+    assert np.abs(got.real - c.real).max() <= 1e-11 and np.abs(got.imag - c.imag).max() <= 1e-11
+
+
+@pytest.mark.parametrize("B", [4, 32])
+def test_idsoft_reads_stale_nyquist_like_reference(ctx, B):
+    N = 2 * B
+    c = O.seeded_coeffs(B, 5)
+    rng = np.random.default_rng(3)
+    stale = rng.standard_normal(N ** 3) + 1j * rng.standard_normal(N ** 3)
+    got = ctx.idsoft(B, c, grid=stale)
+    ref = O.idsoft(B, c, grid=stale)
+    assert rel(got, ref) < TOL
+
+
+def test_dsoft_leaves_input_grid_untouched(ctx, torch):
+    B = 16
+    N = 2 * B
+    g = O.idsoft(B, O.seeded_coeffs(B, 8))
+    gd = torch.from_numpy(g.copy()).cuda()
+    cd = torch.empty(O.coeff_count(B), dtype=torch.complex128, device="cuda")
+    ctx.dsoft_dev(B, gd, cd)
+    ctx.synchronize()
+    assert np.array_equal(gd.cpu().numpy(), g)
+
+
+def test_device_api_on_torch_stream_matches_host_api(ctx, torch):
+    B = 32
+    c = O.seeded_coeffs(B, 11)
+    cd = torch.from_numpy(c).cuda()
+    gd = torch.zeros((2 * B) ** 3, dtype=torch.complex128, device="cuda")
+    s = torch.cuda.current_stream()
+    ctx.idsoft_dev(B, cd, gd, stream=s)
+    back = torch.empty_like(cd)
+    ctx.dsoft_dev(B, gd, back, stream=s)
+    torch.cuda.synchronize()
+    assert np.array_equal(gd.cpu().numpy(), ctx.idsoft(B, c))
+    assert np.array_equal(back.cpu().numpy(), ctx.dsoft(B, gd.cpu().numpy()))
+    assert rel(back.cpu().numpy(), c) < TOL
+
+
+def test_api_mirror_roundtrip_like_reference_example(ctx):
+    # examples/example.cpp:13-85 through the mirrored interface
+    import pfsoftlib_amd as P
+    B = 24
+    sample = P.grid3D(2 * B)
+    coef = P.DSOFTFourierCoefficients(B)
+    rec = P.DSOFTFourierCoefficients(B)
+    coef.mem[:] = O.seeded_coeffs(B, 99)
+    P.IDSOFT(coef, sample, ctx=ctx)
+    P.DSOFT(sample, rec, ctx=ctx)
+    for l in range(B):
+        for M in range(-l, l + 1):
+            for Mp in range(-l, l + 1):
+                d = coef(l, M, Mp) - rec(l, M, Mp)
+                assert abs(d.real) <= 1e-11 and abs(d.imag) <= 1e-11
+
+
+def test_b256_roundtrip_and_linearity(ctx, torch):
+    B = 256
+    n = O.coeff_count(B)
+    import pfsoftlib_amd as P
+    c1 = torch.from_numpy(P.seeded_coeffs(B, 7, 1.5)).cuda()
+    c2 = torch.from_numpy(P.seeded_coeffs(B, 8, 1.5)).cuda()
+    N = 2 * B
+    g = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
+    r = torch.empty(n, dtype=torch.complex128, device="cuda")
+    ctx.idsoft_dev(B, c1, g)
+    ctx.dsoft_dev(B, g, r)
+    ctx.synchronize()
+    assert (torch.linalg.vector_norm(r - c1) / torch.linalg.vector_norm(c1)).item() < TOL
+    # linearity: IDSOFT(c1 + 2 c2) = IDSOFT(c1) + 2 IDSOFT(c2)
+    g1 = g.clone()
+    g.zero_()
+    ctx.idsoft_dev(B, c2, g)
+    g2 = g.clone()
+    g.zero_()
+    c3 = c1 + 2 * c2
+    ctx.idsoft_dev(B, c3, g)
+    ctx.synchronize()
+    lin = g1 + 2 * g2
+    assert (torch.linalg.vector_norm(g - lin) / torch.linalg.vector_norm(lin)).item() < 1e-13
