@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""DSOFT + iDSOFT throughput on one MI355X.
+
+Workload (default): the plain FP64 single-signal round trip at bandwidth
+B = 128 -- seeded coefficients f^l_{M,M'} (0 <= l < 128, uniform[-1,1]/(l+1),
+2,796,160 complex128 values) -> iDSOFT -> 256^3 complex128 grid (268 MB) ->
+DSOFT -> coefficients.  One step = one iDSOFT + one DSOFT on device-resident
+buffers; value = round-trip pairs per second.
+
+Also reported: per-kernel average durations from HIP events recorded on the
+launch stream over the timed region, the roofline figure of the dominant
+kernel, per-step latency percentiles, the round-trip error, and the CPU
+oracle (C/OpenMP restatement of the reference) timed on this host on a
+bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--bandwidth B]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def algorithmic_bytes(kernel, B, e=16):
+    """Algorithmic HBM bytes per launch.  N = 2B, e = bytes per complex value.
+    FFT stage per direction: 2 N^3 e (one read + one write of the grid); the
+    stage is two launches, so each pass is credited half of it.
+    DWT stage per direction: (N-1)^2 N e pencil bytes + B(4B^2-1)/3 e
+    coefficient bytes."""
+    N = 2 * B
+    if kernel.startswith("fft_"):
+        return N ** 3 * e
+    if kernel.startswith("dwt_"):
+        return (N - 1) ** 2 * N * e + B * (4 * B * B - 1) // 3 * e
+    if kernel == "nyquist_gather":
+        return 2 * (2 * N - 1) * N * e
+    return 0
+
+
+def cpu_baseline(B, pairs, seed):
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import numpy as np
+    import oracle_bind as O
+    threads = O.lib().oracle_max_threads()
+    c = O.seeded_coeffs(B, seed)
+    O.dsoft(4, O.idsoft(4, O.seeded_coeffs(4, 1)))  # warm the OpenMP pool
+    t_inv, t_fwd = [], []
+    g = None
+    for _ in range(pairs):
+        t0 = time.perf_counter()
+        g = O.idsoft(B, c, threads=threads)
+        t1 = time.perf_counter()
+        r = O.dsoft(B, g, threads=threads)
+        t2 = time.perf_counter()
+        t_inv.append(t1 - t0)
+        t_fwd.append(t2 - t1)
+    err = float(np.linalg.norm(r - c) / np.linalg.norm(c))
+    tot = sum(t_inv) + sum(t_fwd)
+    return {
+        "value": pairs / tot,
+        "unit": "DSOFT+iDSOFT pairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d full round trips at B=%d (same seeded input), oracle C/OpenMP restatement of the "
+                  "reference (long double DWT, own long double FFT in place of FFTW); fwd avg %.3f s, inv avg %.3f s"
+                  % (pairs, B, sum(t_fwd) / pairs, sum(t_inv) / pairs),
+        "roundtrip_rel_l2": err,
+    }
+
+
+def copy_calibration(torch, steps, mib):
+    """A plain elementwise copy of known size, for scaling PMC byte counters."""
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(steps):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n * 8, "bytes_written": n * 8, "launches": steps}}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--bandwidth", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only (the SO(3) transform of one signal, or a batch, lives "
+              "on one device); --gpus %d / WORLD_SIZE=%d is not supported" % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import pfsoftlib_amd as P
+
+    if args.copy_calibration:
+        copy_calibration(torch, max(args.steps, 1), args.copy_calibration)
+        return
+
+    B = args.bandwidth
+    N = 2 * B
+    ctx = P.Context(0)
+    ctx.warm(B)
+    coeffs = torch.from_numpy(P.seeded_coeffs(B, args.seed)).cuda()
+    grid = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
+    back = torch.empty_like(coeffs)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.idsoft_dev(B, coeffs, grid, stream=stream)
+        ctx.dsoft_dev(B, grid, back, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    P.lib().pfs_timing_enable(ctx.handle, 1)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    P.lib().pfs_timing_enable(ctx.handle, 0)
+
+    import ctypes
+    buf = ctypes.create_string_buffer(1 << 16)
+    rc = P.lib().pfs_timing_collect(ctx.handle, buf, len(buf))
+    if rc != 0:
+        raise RuntimeError(P.lib().pfs_last_error().decode())
+    kernels = {}
+    for line in buf.value.decode().strip().splitlines():
+        name, cnt, tot = line.split(",")
+        cnt, tot = int(cnt), float(tot)
+        avg_s = tot / cnt / 1e3
+        ab = algorithmic_bytes(name, B)
+        kernels[name] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2),
+                         "alg_bytes": ab, "achieved_GBps": round(ab / avg_s / 1e9, 1),
+                         "frac_hbm": round(ab / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    dk = kernels[dom]
+
+    lat = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps))
+    err = (torch.linalg.vector_norm(back - coeffs) / torch.linalg.vector_norm(coeffs)).item()
+    elapsed = t1 - t0
+    out = {
+        "metric": "B=128 DSOFT+iDSOFT/s: FP64 plain vs batched mixed @≤1e-6 err; p99 latency",
+        "value": args.steps / elapsed,
+        "unit": "DSOFT+iDSOFT pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: SplitMix64-seeded coefficients, uniform[-1,1]/(l+1)",
+        "config": {"workload": "plain FP64 single-signal DSOFT+iDSOFT round trip, B=%d, %d^3 complex128 grid, "
+                               "%d coefficients, inputs resident in HBM" % (B, N, P.coeff_count(B)),
+                   "bandwidth": B, "grid": [N, N, N], "precision": "fp64", "signals_per_step": 1},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": None},
+        "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "transforms_per_s": 2 * args.steps / elapsed,
+        "roundtrip_rel_l2": err,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -85,7 +85,10 @@ int pfs_dsoft_f64(pfs_context* ctx, int bandwidth, const double* grid, double* c
 int pfs_idsoft_f64(pfs_context* ctx, int bandwidth, const double* coeffs, double* grid);
 
 /* Same transforms on device buffers (HBM-resident), enqueued on stream
- * (NULL = the context's stream); the calls return once enqueued. */
+ * (a hipStream_t; NULL = the HIP null stream); the calls return once
+ * enqueued.  The pipeline work buffers belong to the context: calls that
+ * overlap in time on different streams of one context race on them, so use
+ * one stream per context (or one context per stream). */
 int pfs_dsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* grid_dev, void* coeffs_dev, void* stream);
 int pfs_idsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* grid_dev, void* stream);
 
@@ -126,11 +129,20 @@ int pfs_stage_dwt_fwd_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
 int pfs_stage_dwt_inv_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* pencils_dev, void* stream);
 int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_dev, void* grid_dev, void* stream);
 
+/* Per-kernel timing (no counterpart in the reference, whose only timer is a
+ * wall clock around whole calls, src/stopwatch.cpp).  While enabled, every
+ * kernel launch of the context is bracketed by HIP events on its launch
+ * stream.  pfs_timing_collect waits for them and writes one CSV line per
+ * kernel, "name,launches,total_ms", into buf, then forgets them. */
+int pfs_timing_enable(pfs_context* ctx, int on);
+int pfs_timing_collect(pfs_context* ctx, char* buf, size_t len);
+
 /* Device memory helpers for callers without their own allocator. */
 int pfs_device_alloc(pfs_context* ctx, size_t bytes, void** out);
 int pfs_device_free(pfs_context* ctx, void* ptr);
 int pfs_memcpy_h2d(pfs_context* ctx, void* dst_dev, const void* src_host, size_t bytes);
 int pfs_memcpy_d2h(pfs_context* ctx, void* dst_host, const void* src_dev, size_t bytes);
+/* Waits for stream; NULL waits for the context's own stream and the null stream. */
 int pfs_synchronize(pfs_context* ctx, void* stream);
 
 #ifdef __cplusplus

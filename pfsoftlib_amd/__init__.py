@@ -64,6 +64,8 @@ def lib():
             "pfs_stage_dwt_fwd_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_stage_dwt_inv_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_stage_fft_inv_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_timing_enable": (ip, [vp, ip]),
+            "pfs_timing_collect": (ip, [vp, ctypes.c_char_p, ctypes.c_size_t]),
             "pfs_device_alloc": (ip, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
             "pfs_device_free": (ip, [vp, vp]),
             "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),

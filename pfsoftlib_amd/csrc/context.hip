@@ -123,7 +123,7 @@ __global__ void k_wigner_table(int B, int M, int Mp, const double* cosb, const d
 
 // Self-test of the FP64 MFMA lane layout the DWT kernels rely on:
 // A (16x4): lane i + 16kk holds A[i][kk]; B (4x16): lane j + 16kk holds
-// B[kk][j]; D (16x16): lane L holds D[4(L/16) + v][L%16].  With a single
+// B[kk][j]; D (16x16): lane L holds D[4v + L/16][L%16].  With a single
 // non-zero K index ksel, A[i][ksel] = 2^i and B[ksel][j] = 3^j, every output is
 // 2^i 3^j, which names its row and column uniquely.
 __global__ void k_mfma_probe(double* out)
@@ -159,7 +159,7 @@ int mfma_selftest(hipStream_t st)
                 int i = 0, j = 0;
                 while (x > 0 && fmod(x, 2.0) == 0.0) { x /= 2.0; ++i; }
                 while (x > 1 && fmod(x, 3.0) == 0.0) { x /= 3.0; ++j; }
-                int ei = 4 * (lane >> 4) + v, ej = lane & 15;
+                int ei = 4 * v + (lane >> 4), ej = lane & 15;
                 if (x != 1.0 || i != ei || j != ej)
                     return fail(PFS_ERR_UNSUPPORTED,
                                 "MFMA f64 16x16x4 layout differs from the assumed one: ksel=%d lane=%d v=%d gives "
@@ -195,9 +195,40 @@ struct pfs_context {
     size_t io_coeff_bytes = 0;
     // allocations handed out by pfs_device_alloc
     std::map<void*, size_t> user_allocs;
+    // per-kernel timing: a start/stop event pair recorded around every launch
+    // on the launch stream while enabled
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { const char* name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
 };
 
 namespace {
+
+hipEvent_t take_event(pfs_context* c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Wraps one kernel launch: records events around it when timing is enabled.
+template <typename F>
+hipError_t timed(pfs_context* c, const char* name, hipStream_t st, F&& launch)
+{
+    if (!c->timing) return launch();
+    hipEvent_t a = take_event(c), b = take_event(c);
+    if (a) (void)hipEventRecord(a, st);
+    hipError_t e = launch();
+    if (b) (void)hipEventRecord(b, st);
+    if (a && b) c->recs.push_back({name, a, b});
+    return e;
+}
 
 int ctx_alloc(pfs_context* c, void** p, size_t bytes)
 {
@@ -298,7 +329,8 @@ int get_plan(pfs_context* c, int B, Plan** out)
 int ensure_work(pfs_context* c, size_t bytes)
 {
     if (c->work_bytes >= bytes) return PFS_OK;
-    (void)hipStreamSynchronize(c->stream);
+    // the old buffers may still be in use on any stream a caller passed
+    (void)hipDeviceSynchronize();
     ctx_free(c, c->work[0], c->work_bytes);
     ctx_free(c, c->work[1], c->work_bytes);
     c->work_bytes = 0;
@@ -371,13 +403,18 @@ LineMap map_cols_seam(int N)
     return m;
 }
 
-hipStream_t pick(pfs_context* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+// Device entry points take a caller stream; NULL is the HIP null stream.
+hipStream_t pick(pfs_context* c, void* s)
+{
+    (void)c;
+    return (hipStream_t)s;
+}
 
 int fft_fwd(pfs_context* c, const Plan& p, const double2* grid, double2* W, double2* G, hipStream_t st)
 {
     const int N = p.t.N;
-    HIP_TRY(launch_fft_lines_f64(grid, W, map_rows_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
-    HIP_TRY(launch_fft_lines_f64(W, G, map_cols_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    HIP_TRY(timed(c, "fft_fwd_rows", st, [&] { return launch_fft_lines_f64(grid, W, map_rows_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st); }));
+    HIP_TRY(timed(c, "fft_fwd_cols", st, [&] { return launch_fft_lines_f64(W, G, map_cols_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st); }));
     return PFS_OK;
 }
 
@@ -385,9 +422,10 @@ int fft_inv(pfs_context* c, const Plan& p, const double2* G, double2* W, double2
 {
     const int N = p.t.N;
     const int B = p.B;
-    HIP_TRY(launch_fft_lines_f64(G, W, map_cols_inv(N), N, +1, 1.0, p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st));
-    HIP_TRY(launch_fft_lines_f64(W, grid, map_rows_inv(N), N, +1, 1. / (4. * B * B), p.t.tw, LINES_TILE_IN_ROW_OUT,
-                                 1, st));
+    HIP_TRY(timed(c, "fft_inv_cols", st, [&] { return launch_fft_lines_f64(G, W, map_cols_inv(N), N, +1, 1.0, p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st); }));
+    HIP_TRY(timed(c, "fft_inv_rows", st, [&] {
+        return launch_fft_lines_f64(W, grid, map_rows_inv(N), N, +1, 1. / (4. * B * B), p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st);
+    }));
     return PFS_OK;
 }
 
@@ -406,7 +444,7 @@ int dsoft_dev(pfs_context* c, int B, const double2* grid, double2* coeffs, hipSt
     double2* G = (double2*)c->work[1];
     rc = fft_fwd(c, *p, grid, W, G, st);
     if (rc) return rc;
-    HIP_TRY(launch_dwt_fwd_f64(p->t, G, coeffs, fwd_norm(B), st));
+    HIP_TRY(timed(c, "dwt_fwd", st, [&] { return launch_dwt_fwd_f64(p->t, G, coeffs, fwd_norm(B), st); }));
     return PFS_OK;
 }
 
@@ -419,8 +457,8 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     if (rc) return rc;
     double2* W = (double2*)c->work[0];
     double2* G = (double2*)c->work[1];
-    HIP_TRY(launch_nyquist_gather_f64(grid, G, B, st));
-    HIP_TRY(launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st));
+    HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st); }));
+    HIP_TRY(timed(c, "dwt_inv", st, [&] { return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st); }));
     return fft_inv(c, *p, G, W, grid, st);
 }
 
@@ -529,6 +567,13 @@ int pfs_ctx_destroy(pfs_context* c)
             ctx_free(c, ptr, kv.second);
         }
         c->user_allocs.clear();
+        for (auto& r : c->recs) {
+            c->ev_pool.push_back(r.a);
+            c->ev_pool.push_back(r.b);
+        }
+        c->recs.clear();
+        for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+        c->ev_pool.clear();
         (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -708,6 +753,42 @@ int pfs_stage_fft_inv_dev(pfs_context* c, int B, const void* pencils_dev, void* 
     return fft_inv(c, *p, (const double2*)pencils_dev, (double2*)c->work[0], (double2*)grid_dev, pick(c, stream));
 }
 
+int pfs_timing_enable(pfs_context* c, int on)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->timing = on != 0;
+    return PFS_OK;
+}
+
+int pfs_timing_collect(pfs_context* c, char* buf, size_t len)
+{
+    if (!c || !buf || len == 0) return fail(PFS_ERR_ARG, "pfs_timing_collect: bad arguments");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    std::map<std::string, std::pair<long, double>> acc;
+    for (auto& r : c->recs) {
+        HIP_TRY(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        auto& e = acc[r.name];
+        e.first += 1;
+        e.second += ms;
+        c->ev_pool.push_back(r.a);
+        c->ev_pool.push_back(r.b);
+    }
+    c->recs.clear();
+    std::string out;
+    for (auto& kv : acc) {
+        char line[160];
+        snprintf(line, sizeof(line), "%s,%ld,%.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+        out += line;
+    }
+    if (out.size() + 1 > len) return fail(PFS_ERR_ARG, "pfs_timing_collect: buffer too small");
+    memcpy(buf, out.c_str(), out.size() + 1);
+    return PFS_OK;
+}
+
 int pfs_device_alloc(pfs_context* c, size_t bytes, void** out)
 {
     if (!c || !out) return fail(PFS_ERR_ARG, "null argument");
@@ -752,7 +833,12 @@ int pfs_synchronize(pfs_context* c, void* stream)
 {
     if (!c) return fail(PFS_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipStreamSynchronize(pick(c, stream)));
+    if (stream) {
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    } else {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipStreamSynchronize(nullptr));
+    }
     return PFS_OK;
 }
 

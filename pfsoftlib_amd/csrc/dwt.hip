@@ -106,8 +106,9 @@ __device__ __forceinline__ void recurrence_coeffs(int M, int Mp, int r, int R, d
     cf[2 * stride + r] = c1;
 }
 
-// MFMA f64 16x16x4 output layout: lane L holds D[4*(L/16) + v][L%16], v < 4.
-__device__ __forceinline__ int mfma_out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
+// MFMA f64 16x16x4 output layout (gfx950, checked by the context's
+// self-test): lane L holds D[4v + L/16][L%16] in accumulator register v < 4.
+__device__ __forceinline__ int mfma_out_row(int lane, int v) { return 4 * v + (lane >> 4); }
 __device__ __forceinline__ int mfma_out_col(int lane) { return lane & 15; }
 
 __global__ __launch_bounds__(1024) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,

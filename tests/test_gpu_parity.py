@@ -77,14 +77,15 @@ def test_fft_stage_pencil_layout(ctx, torch, B):
     g = rng.standard_normal(N ** 3) + 1j * rng.standard_normal(N ** 3)
     gd = torch.from_numpy(g).cuda()
     pen = torch.empty_like(gd)
-    ctx.stage_fft_fwd(B, gd, pen)
-    ctx.synchronize()
+    s = torch.cuda.current_stream()
+    ctx.stage_fft_fwd(B, gd, pen, stream=s)
+    torch.cuda.synchronize()
     F = O.fft2_layers(g.reshape(N, N, N), -1)           # [k][q][p]
     ref = np.transpose(F, (2, 1, 0)).reshape(-1)         # [p][q][k]
     assert rel(pen.cpu().numpy(), ref) < 1e-14
     back = torch.empty_like(gd)
-    ctx.stage_fft_inv(B, pen, back)
-    ctx.synchronize()
+    ctx.stage_fft_inv(B, pen, back, stream=s)
+    torch.cuda.synchronize()
     assert rel(back.cpu().numpy(), g) < 1e-13
 
 
@@ -141,8 +142,8 @@ def test_dsoft_leaves_input_grid_untouched(ctx, torch):
     g = O.idsoft(B, O.seeded_coeffs(B, 8))
     gd = torch.from_numpy(g.copy()).cuda()
     cd = torch.empty(O.coeff_count(B), dtype=torch.complex128, device="cuda")
-    ctx.dsoft_dev(B, gd, cd)
-    ctx.synchronize()
+    ctx.dsoft_dev(B, gd, cd, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
     assert np.array_equal(gd.cpu().numpy(), g)
 
 
@@ -179,26 +180,41 @@ def test_api_mirror_roundtrip_like_reference_example(ctx):
 
 
 def test_b256_roundtrip_and_linearity(ctx, torch):
+    # every device call runs on torch's current stream, so torch's own ops
+    # (zero_, clone, arithmetic) are ordered with the transforms
     B = 256
     n = O.coeff_count(B)
     import pfsoftlib_amd as P
+    s = torch.cuda.current_stream()
     c1 = torch.from_numpy(P.seeded_coeffs(B, 7, 1.5)).cuda()
     c2 = torch.from_numpy(P.seeded_coeffs(B, 8, 1.5)).cuda()
     N = 2 * B
     g = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
     r = torch.empty(n, dtype=torch.complex128, device="cuda")
-    ctx.idsoft_dev(B, c1, g)
-    ctx.dsoft_dev(B, g, r)
-    ctx.synchronize()
+    ctx.idsoft_dev(B, c1, g, stream=s)
+    ctx.dsoft_dev(B, g, r, stream=s)
+    torch.cuda.synchronize()
     assert (torch.linalg.vector_norm(r - c1) / torch.linalg.vector_norm(c1)).item() < TOL
     # linearity: IDSOFT(c1 + 2 c2) = IDSOFT(c1) + 2 IDSOFT(c2)
     g1 = g.clone()
     g.zero_()
-    ctx.idsoft_dev(B, c2, g)
+    ctx.idsoft_dev(B, c2, g, stream=s)
     g2 = g.clone()
     g.zero_()
     c3 = c1 + 2 * c2
-    ctx.idsoft_dev(B, c3, g)
-    ctx.synchronize()
+    ctx.idsoft_dev(B, c3, g, stream=s)
     lin = g1 + 2 * g2
+    torch.cuda.synchronize()
     assert (torch.linalg.vector_norm(g - lin) / torch.linalg.vector_norm(lin)).item() < 1e-13
+
+
+def test_cpp_dropin_example_like_reference(ctx):
+    # tests/cpp/example_roundtrip.cpp mirrors examples/example.cpp through include/pfsoft_amd.hpp
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "example_roundtrip")
+    r = subprocess.run([exe, "32"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "Correct result:  Yes" in r.stdout
+    assert "[PFSOFTLib warning]" in r.stderr and "not even" in r.stderr
