@@ -1,11 +1,13 @@
 // Discrete Wigner transform stage, FP64, Wigner-d generated on the fly.
 //
-// One workgroup per canonical pair (M, M'), M >= M' >= 0, with one wave per
-// 64 beta nodes.  Every lane runs the three-term recurrence of the
-// L2-normalised Wigner d-function for its own node k in FP64 registers
-// (fn_dwt.hpp:152-199: base row, then rows l = M+1 .. B-1), 16 degrees at a
-// time, into a per-wave LDS tile.  The tile feeds v_mfma_f64_16x16x4_f64
-// against the 16 real right-hand sides of the pair: the 8 symmetric cases
+// One wave per canonical pair (M, M'), M >= M' >= 0 (two or four waves for
+// 2B >= 512).  Lane i of the wave runs the three-term recurrence of the
+// L2-normalised Wigner d-function (fn_dwt.hpp:152-199: base row, then rows
+// l = M+1 .. B-1) in FP64 registers for the beta nodes k = i, i+64, ... --
+// one recurrence chain per 64 nodes.  Sixteen degrees of one chain at a time
+// go through a 16 x 64 LDS tile (ping-pong buffered) into the
+// v_mfma_f64_16x16x4_f64 operand layout; the MFMAs contract them against the
+// 16 real right-hand sides of the pair.  The 8 symmetric cases
 // (l, +-M, +-M') / (l, +-M', +-M) of the reference (src/fn_dsoft.cpp:112-283,
 // src/fn_idsoft.cpp:100-283) share one table D+ and differ only in which
 // (p, q) beta-vector they read, whether that vector is read reversed (the
@@ -25,16 +27,16 @@
 //
 // The reference builds the table with base sign -1 and undoes it per case
 // (fn_dwt.hpp:124-125); here D+ carries the positive base and the table
-// above already holds the net sign.  The forward weights w_k multiply the
-// input vectors instead of the table (w is symmetric, so reversed cases see
-// the same weight).
+// above already holds the net sign.  The forward transform uses the weighted
+// table: like weighted_wigner_d_matrix (fn_dwt.hpp:162) the quadrature weight
+// multiplies the base row and propagates through the linear recurrence.
 #include "pfs_internal.h"
 
 namespace pfs {
 
-constexpr int TSTR = 68;          // LDS tile row stride (doubles): 64 nodes + 4 pad
-constexpr int OSTR = 18;          // output staging row stride (doubles)
-constexpr int WAVE_LDS = 1152;    // doubles per wave: max(16*TSTR, 64*OSTR)
+constexpr int TSTR = 66;            // LDS tile row stride (doubles): 64 nodes + 2 pad
+constexpr int TILE = 16 * TSTR;     // one 16-degree x 64-node tile
+constexpr int OSTR = 18;            // output staging row stride (doubles), 64 rows
 
 struct CaseInfo {
     long vec;     // pencil offset (complex elements) of the (p, q) beta-vector
@@ -111,8 +113,52 @@ __device__ __forceinline__ void recurrence_coeffs(int M, int Mp, int r, int R, d
 __device__ __forceinline__ int mfma_out_row(int lane, int v) { return 4 * v + (lane >> 4); }
 __device__ __forceinline__ int mfma_out_col(int lane) { return lane & 15; }
 
-__global__ __launch_bounds__(1024) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,
-                                                      double2* __restrict__ coeffs, double norm)
+// Per-lane recurrence chains: chain c covers node k = kbase + 64 c.
+template <int NCH>
+struct Chains {
+    double cb[NCH], cur[NCH], prev[NCH];
+};
+
+template <int NCH>
+__device__ __forceinline__ void init_chains(Chains<NCH>& ch, const PlanTables& t, int M, int Mp, int kbase,
+                                            bool weighted)
+{
+    const double nf = t.normfac[M * t.B + Mp];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        int k = kbase + 64 * c;
+        bool kv = k < t.N;
+        ch.cb[c] = kv ? t.cosb[k] : 0.0;
+        double b = kv ? nf * ipow(t.sinh[k], M - Mp) * ipow(t.cosh[k], M + Mp) : 0.0;
+        if (weighted && kv) b *= t.wts[k];
+        ch.cur[c] = b;
+        ch.prev[c] = 0.0;
+    }
+}
+
+// Writes degrees r0 .. r0+15 of chain c into the tile (row rr, column lane)
+// and advances the chain past them.
+template <int NCH>
+__device__ __forceinline__ void run_chain(Chains<NCH>& ch, int c, double* tile, int lane, const double* cf, int RP,
+                                          int r0)
+{
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+        int r = r0 + rr;
+        tile[rr * TSTR + lane] = ch.cur[c];
+        double u = fma(cf[r], ch.cb[c], cf[RP + r]);
+        double dn = fma(ch.cur[c], u, cf[2 * RP + r] * ch.prev[c]);
+        ch.prev[c] = ch.cur[c];
+        ch.cur[c] = dn;
+    }
+}
+
+// Forward DWT.  NCH chains per lane (64 nodes each), WPP waves per pair
+// splitting the nodes; KQ = nodes per chain that carry data (64, or N
+// rounded up to 4 when N < 64).
+template <int NCH, int WPP>
+__global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,
+                                                          double2* __restrict__ coeffs, double norm)
 {
     extern __shared__ __align__(16) double sm[];
     const int B = t.B, N = t.N;
@@ -121,79 +167,90 @@ __global__ __launch_bounds__(1024) void k_dwt_fwd_f64(PlanTables t, const double
     const int R = B - M;
     const int nch = (R + 15) >> 4;
     const int RP = nch * 16;
-    const int nthr = blockDim.x, waves = nthr >> 6;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int KQ = N >= 64 ? 64 : ((N + 3) & ~3);
+    const int nmf = KQ >> 2;  // MFMAs per chain and chunk
 
-    double* tile = sm + w * WAVE_LDS;
-    double* red = sm + waves * WAVE_LDS;       // waves * 256
-    double* cf = red + waves * 256;            // 3 * RP
+    double* tiles = sm + w * 2 * TILE;
+    double* red = sm + WPP * 2 * TILE;   // WPP * 256 (only when WPP > 1)
+    double* cf = red + (WPP > 1 ? WPP * 256 : 0);
 
-    for (int r = tid; r < RP; r += nthr) recurrence_coeffs(M, Mp, r, R, cf, RP);
+    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
 
-    // this lane's beta node for the recurrence
-    const int k = 64 * w + lane;
-    const bool kv = k < N;
-    const double cb = kv ? t.cosb[k] : 0.0;
-    double d_cur = kv ? t.normfac[M * B + Mp] * ipow(t.sinh[k], M - Mp) * ipow(t.cosh[k], M + Mp) : 0.0;
-    double d_prev = 0.0;
+    const int kbase = w * 64 * NCH + lane;
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, t, M, Mp, kbase, true);
 
-    // B operand: lane (j + 16 kk) holds S[64w + 4m + kk][j], j = 2 case + re/im
+    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
     const int j = lane & 15, kk = lane >> 4;
     const CaseInfo ci = case_info(j >> 1, M, Mp, N);
     const double* vsrc = reinterpret_cast<const double*>(G + ci.vec) + (j & 1);
-    double sreg[16];
+    double sreg[NCH * 16];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        int ks = 64 * w + 4 * m + kk;
+    for (int m = 0; m < NCH * 16; ++m) {
+        int ks = w * 64 * NCH + 4 * m + kk;
         double v = 0.0;
-        if (ci.active && ks < N) {
+        if (ci.active && ks < N && ((4 * m) & 63) < KQ) {
             int kr = ci.flip ? N - 1 - ks : ks;
-            v = vsrc[2 * kr] * t.wts[ks];
+            v = vsrc[2 * kr];
         }
         sreg[m] = v;
     }
     __syncthreads();
 
-    for (int ch = 0; ch < nch; ++ch) {
+    for (int chk = 0; chk < nch; ++chk) {
+        v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {
-            int r = ch * 16 + rr;
-            tile[rr * TSTR + lane] = d_cur;
-            double u = fma(cf[r], cb, cf[RP + r]);
-            double dn = fma(d_cur, u, cf[2 * RP + r] * d_prev);
-            d_prev = d_cur;
-            d_cur = dn;
-        }
-        __syncthreads();
-        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < NCH; ++c) {
+            double* tile = tiles + (c & 1) * TILE;
+            run_chain<NCH>(ch, c, tile, lane, cf, RP, chk * 16);
+            __syncthreads();
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            double a = tile[(lane & 15) * TSTR + 4 * m + (lane >> 4)];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[m], acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) red[w * 256 + mfma_out_row(lane, v) * 16 + mfma_out_col(lane)] = acc[v];
-        __syncthreads();
-        for (int e = tid; e < 256; e += nthr) {
-            double s = 0.0;
-            for (int ww = 0; ww < waves; ++ww) s += red[ww * 256 + e];
-            int i = e >> 4, jj = e & 15;
-            int r = ch * 16 + i;
-            CaseInfo co = case_info(jj >> 1, M, Mp, N);
-            if (r < R && co.active) {
-                int l = M + r;
-                double sg = (co.flip && (r & 1)) ? -co.sgn : co.sgn;
-                reinterpret_cast<double*>(coeffs + coeff_index(l, co.Mc, co.Mpc))[jj & 1] = norm * (sg * s);
+            for (int m = 0; m < 16; ++m) {
+                if (m < nmf) {
+                    double a = tile[(lane & 15) * TSTR + 4 * m + (lane >> 4)];
+                    if (m & 1) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[16 * c + m], acc1, 0, 0, 0);
+                    else acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[16 * c + m], acc0, 0, 0, 0);
+                }
             }
         }
-        // the next chunk writes tile (read above before the barrier) and red
-        // (read by this loop, which every thread finishes before the next
-        // chunk's first barrier)
+        v4d acc = acc0 + acc1;
+        if (WPP > 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[w * 256 + mfma_out_row(lane, v) * 16 + mfma_out_col(lane)] = acc[v];
+            __syncthreads();
+            if (w == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double s = 0.0;
+                    for (int ww = 0; ww < WPP; ++ww) s += red[ww * 256 + mfma_out_row(lane, v) * 16 + mfma_out_col(lane)];
+                    acc[v] = s;
+                }
+            }
+            __syncthreads();
+        }
+        if (w == 0) {
+            const int jj = mfma_out_col(lane);
+            const CaseInfo co = case_info(jj >> 1, M, Mp, N);
+            if (co.active) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int r = chk * 16 + mfma_out_row(lane, v);
+                    if (r < R) {
+                        double sg = (co.flip && (r & 1)) ? -co.sgn : co.sgn;
+                        reinterpret_cast<double*>(coeffs + coeff_index(M + r, co.Mc, co.Mpc))[jj & 1] =
+                            norm * (sg * acc[v]);
+                    }
+                }
+            }
+        }
     }
 }
 
-__global__ __launch_bounds__(1024) void k_dwt_inv_f64(PlanTables t, const double2* __restrict__ coeffs,
-                                                      double2* __restrict__ G, double norm)
+// Inverse DWT: every wave owns its nodes outright (no reduction).
+template <int NCH, int WPP>
+__global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const double2* __restrict__ coeffs,
+                                                          double2* __restrict__ G, double norm)
 {
     extern __shared__ __align__(16) double sm[];
     const int B = t.B, N = t.N;
@@ -202,101 +259,147 @@ __global__ __launch_bounds__(1024) void k_dwt_inv_f64(PlanTables t, const double
     const int R = B - M;
     const int nch = (R + 15) >> 4;
     const int RP = nch * 16;
-    const int nthr = blockDim.x, waves = nthr >> 6;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int KQ = N >= 64 ? 64 : ((N + 3) & ~3);
+    const int nkb = KQ >= 16 ? 4 : 1;  // 16-node blocks per chain
 
-    double* tile = sm + w * WAVE_LDS;
-    double* cv = sm + waves * WAVE_LDS;   // RP * 16
-    double* cf = cv + RP * 16;            // 3 * RP
+    double* tiles = sm + w * 2 * TILE;
+    double* cf = sm + WPP * 2 * TILE;
 
-    for (int r = tid; r < RP; r += nthr) recurrence_coeffs(M, Mp, r, R, cf, RP);
-    // coefficient right-hand sides, with norm and the per-case signs applied
-    for (int e = tid; e < RP * 16; e += nthr) {
-        int r = e >> 4, jj = e & 15;
-        CaseInfo co = case_info(jj >> 1, M, Mp, N);
-        double v = 0.0;
-        if (r < R && co.active) {
-            double sg = (co.flip && (r & 1)) ? -co.sgn : co.sgn;
-            v = sg * (norm * reinterpret_cast<const double*>(coeffs + coeff_index(M + r, co.Mc, co.Mpc))[jj & 1]);
-        }
-        cv[e] = v;
-    }
+    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
 
-    const int k = 64 * w + lane;
-    const bool kv = k < N;
-    const double cb = kv ? t.cosb[k] : 0.0;
-    double d_cur = kv ? t.normfac[M * B + Mp] * ipow(t.sinh[k], M - Mp) * ipow(t.cosh[k], M + Mp) : 0.0;
-    double d_prev = 0.0;
+    const int kw = w * 64 * NCH;
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, t, M, Mp, kw + lane, false);
 
-    v4d acc[4];
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc[kb] = (v4d){0.0, 0.0, 0.0, 0.0};
-    __syncthreads();
-
-    for (int ch = 0; ch < nch; ++ch) {
-#pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {
-            int r = ch * 16 + rr;
-            tile[rr * TSTR + lane] = d_cur;
-            double u = fma(cf[r], cb, cf[RP + r]);
-            double dn = fma(d_cur, u, cf[2 * RP + r] * d_prev);
-            d_prev = d_cur;
-            d_cur = dn;
-        }
-        __syncthreads();
+    // B operand rows: lane (j + 16 b) holds n' * sign * coeff(l = M + r0 + 4s + b, case j/2).(re|im)
+    const int j = lane & 15, bsel = lane >> 4;
+    const CaseInfo ci = case_info(j >> 1, M, Mp, N);
+    auto load_b = [&](int chk, double* b) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            double b = cv[(ch * 16 + 4 * s + (lane >> 4)) * 16 + (lane & 15)];
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
-                double a = tile[(4 * s + (lane >> 4)) * TSTR + 16 * kb + (lane & 15)];
-                acc[kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[kb], 0, 0, 0);
+            int r = chk * 16 + 4 * s + bsel;
+            double v = 0.0;
+            if (ci.active && r < R) {
+                double sg = (ci.flip && (r & 1)) ? -ci.sgn : ci.sgn;
+                v = sg * (norm * reinterpret_cast<const double*>(coeffs + coeff_index(M + r, ci.Mc, ci.Mpc))[j & 1]);
             }
+            b[s] = v;
         }
-        __syncthreads();
+    };
+
+    v4d acc[NCH * 4];
+#pragma unroll
+    for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double bcur[4], bnext[4];
+    load_b(0, bcur);
+    __syncthreads();
+
+    for (int chk = 0; chk < nch; ++chk) {
+        if (chk + 1 < nch) load_b(chk + 1, bnext);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            double* tile = tiles + (c & 1) * TILE;
+            run_chain<NCH>(ch, c, tile, lane, cf, RP, chk * 16);
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+                    if (kb < nkb) {
+                        double a = tile[(4 * s + bsel) * TSTR + 16 * kb + j];
+                        acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[s], acc[4 * c + kb], 0, 0, 0);
+                    }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bcur[s] = bnext[s];
     }
 
-    // stage this wave's 64 x 16 block and write each case's beta-vector run
+    // stage each chain's 64 x 16 block and write each case's beta-vector run
+    double* stage = tiles;  // 64 * OSTR <= 2 * TILE
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
+    for (int c = 0; c < NCH; ++c) {
+        __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-            tile[(16 * kb + mfma_out_row(lane, v)) * OSTR + mfma_out_col(lane)] = acc[kb][v];
-    __syncthreads();
-    if (kv) {
-        for (int c = 0; c < 8; ++c) {
-            CaseInfo co = case_info(c, M, Mp, N);
-            if (!co.active) continue;
-            double2 val;
-            val.x = tile[lane * OSTR + 2 * c];
-            val.y = tile[lane * OSTR + 2 * c + 1];
-            int kr = co.flip ? N - 1 - k : k;
-            G[co.vec + kr] = val;
+        for (int kb = 0; kb < 4; ++kb)
+            if (kb < nkb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    stage[(16 * kb + mfma_out_row(lane, v)) * OSTR + mfma_out_col(lane)] = acc[4 * c + kb][v];
+        __syncthreads();
+        const int k = kw + 64 * c + lane;
+        if (k < N) {
+            for (int cs = 0; cs < 8; ++cs) {
+                CaseInfo co = case_info(cs, M, Mp, N);
+                if (!co.active) continue;
+                double2 val;
+                val.x = stage[lane * OSTR + 2 * cs];
+                val.y = stage[lane * OSTR + 2 * cs + 1];
+                G[co.vec + (co.flip ? N - 1 - k : k)] = val;
+            }
         }
     }
 }
 
-static int dwt_waves(int N) { return (N + 63) / 64; }
+struct DwtShape {
+    int nch, wpp;
+};
+
+// Forward: up to 4 chains per lane in one wave (the cross-wave reduction is
+// paid only above 256 nodes).  Inverse: 2 chains per lane and one wave per
+// 128 nodes -- its waves need no reduction and fewer accumulators keep more
+// waves resident.
+static DwtShape dwt_shape(int N, bool fwd)
+{
+    if (N <= 64) return {1, 1};
+    if (N <= 128) return {2, 1};
+    if (fwd) {
+        if (N <= 256) return {4, 1};
+        if (N <= 512) return {4, 2};
+        return {4, 4};
+    }
+    if (N <= 256) return {2, 2};
+    if (N <= 512) return {2, 4};
+    return {2, 8};
+}
+
+static size_t dwt_smem(int B, int N, bool fwd)
+{
+    DwtShape s = dwt_shape(N, fwd);
+    int RP = ((B + 15) / 16) * 16;
+    size_t d = (size_t)s.wpp * 2 * TILE + 3 * RP;
+    if (fwd && s.wpp > 1) d += (size_t)s.wpp * 256;
+    return d * sizeof(double);
+}
+
+#define PFS_DWT_LAUNCH(KERNEL, NCH, WPP, ...) \
+    hipLaunchKernelGGL((KERNEL<NCH, WPP>), dim3(t.npairs), dim3(64 * WPP), smem, st, __VA_ARGS__)
 
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st)
 {
-    int waves = dwt_waves(t.N);
-    if (waves > 16) return hipErrorInvalidValue;
-    int RP = ((t.B + 15) / 16) * 16;
-    size_t smem = sizeof(double) * ((size_t)waves * WAVE_LDS + waves * 256 + 3 * RP);
+    if (t.N > 1024) return hipErrorInvalidValue;
+    DwtShape s = dwt_shape(t.N, true);
+    size_t smem = dwt_smem(t.B, t.N, true);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k_dwt_fwd_f64, dim3(t.npairs), dim3(64 * waves), smem, st, t, G, coeffs, norm);
+    if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 1, 1, t, G, coeffs, norm);
+    else if (s.nch == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 2, 1, t, G, coeffs, norm);
+    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 1, t, G, coeffs, norm);
+    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 2, t, G, coeffs, norm);
+    else PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 4, t, G, coeffs, norm);
     return hipGetLastError();
 }
 
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st)
 {
-    int waves = dwt_waves(t.N);
-    if (waves > 16) return hipErrorInvalidValue;
-    int RP = ((t.B + 15) / 16) * 16;
-    size_t smem = sizeof(double) * ((size_t)waves * WAVE_LDS + (size_t)RP * 16 + 3 * RP);
+    if (t.N > 1024) return hipErrorInvalidValue;
+    DwtShape s = dwt_shape(t.N, false);
+    size_t smem = dwt_smem(t.B, t.N, false);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k_dwt_inv_f64, dim3(t.npairs), dim3(64 * waves), smem, st, t, coeffs, G, norm);
+    if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 1, 1, t, coeffs, G, norm);
+    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 1, t, coeffs, G, norm);
+    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 2, t, coeffs, G, norm);
+    else if (s.wpp == 4) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 4, t, coeffs, G, norm);
+    else PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 8, t, coeffs, G, norm);
     return hipGetLastError();
 }
 
