@@ -24,10 +24,11 @@ import time
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def algorithmic_bytes(kernel, B, e=16):
-    """Algorithmic HBM bytes per launch.  N = 2B, e = bytes per complex value.
-    FFT stage per direction: 2 N^3 e (one read + one write of the grid); the
-    stage is two launches, so each pass is credited half of it.
+def algorithmic_bytes_per_step(kernel, B, e=16):
+    """Algorithmic HBM bytes one transform moves through this kernel (summed
+    over its launches).  N = 2B, e = bytes per complex value.
+    FFT stage per direction: 2 N^3 e (one read + one write of the grid),
+    split evenly between its row and column passes.
     DWT stage per direction: (N-1)^2 N e pencil bytes + B(4B^2-1)/3 e
     coefficient bytes."""
     N = 2 * B
@@ -92,6 +93,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--fft-chunk", type=int, default=0, help="beta-slices per FFT chunk (0 = automatic)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -112,6 +114,7 @@ def main():
     N = 2 * B
     ctx = P.Context(0)
     ctx.warm(B)
+    ctx.tune("fft_chunk_slices", args.fft_chunk)
     coeffs = torch.from_numpy(P.seeded_coeffs(B, args.seed)).cuda()
     grid = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
     back = torch.empty_like(coeffs)
@@ -125,7 +128,7 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    P.lib().pfs_timing_enable(ctx.handle, 1)
+    ctx.timing(True)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -135,23 +138,17 @@ def main():
         ev[i][1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    P.lib().pfs_timing_enable(ctx.handle, 0)
+    ctx.timing(False)
 
-    import ctypes
-    buf = ctypes.create_string_buffer(1 << 16)
-    rc = P.lib().pfs_timing_collect(ctx.handle, buf, len(buf))
-    if rc != 0:
-        raise RuntimeError(P.lib().pfs_last_error().decode())
     kernels = {}
-    for line in buf.value.decode().strip().splitlines():
-        name, cnt, tot = line.split(",")
-        cnt, tot = int(cnt), float(tot)
-        avg_s = tot / cnt / 1e3
-        ab = algorithmic_bytes(name, B)
-        kernels[name] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2),
-                         "alg_bytes": ab, "achieved_GBps": round(ab / avg_s / 1e9, 1),
-                         "frac_hbm": round(ab / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    for name, (cnt, tot_ms) in ctx.timing_collect().items():
+        per_step_s = tot_ms / args.steps / 1e3
+        ab = algorithmic_bytes_per_step(name, B)
+        kernels[name] = {"launches_per_step": cnt // args.steps, "us_per_step": round(per_step_s * 1e6, 2),
+                         "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "alg_bytes_per_step": ab,
+                         "achieved_GBps": round(ab / per_step_s / 1e9, 1),
+                         "frac_hbm": round(ab / per_step_s / 1e9 / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
     dk = kernels[dom]
 
     lat = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps))

@@ -129,6 +129,13 @@ int pfs_stage_dwt_fwd_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
 int pfs_stage_dwt_inv_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* pencils_dev, void* stream);
 int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_dev, void* grid_dev, void* stream);
 
+/* Tuning knobs (no counterpart in the reference).  Keys:
+ *   "fft_chunk_slices"  beta-slices per FFT chunk (0 = automatic, ~32 MiB of
+ *                       slices so the intermediate stays in the Infinity
+ *                       Cache); rounded to a power of two dividing 2B. */
+int pfs_tune_set(pfs_context* ctx, const char* key, long value);
+int pfs_tune_get(pfs_context* ctx, const char* key, long* value);
+
 /* Per-kernel timing (no counterpart in the reference, whose only timer is a
  * wall clock around whole calls, src/stopwatch.cpp).  While enabled, every
  * kernel launch of the context is bracketed by HIP events on its launch

@@ -64,6 +64,8 @@ def lib():
             "pfs_stage_dwt_fwd_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_stage_dwt_inv_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_stage_fft_inv_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_tune_set": (ip, [vp, ctypes.c_char_p, ctypes.c_long]),
+            "pfs_tune_get": (ip, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]),
             "pfs_timing_enable": (ip, [vp, ip]),
             "pfs_timing_collect": (ip, [vp, ctypes.c_char_p, ctypes.c_size_t]),
             "pfs_device_alloc": (ip, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
@@ -141,6 +143,22 @@ class Context:
 
     def warm(self, B):
         _check(lib().pfs_plan_warm(self._h, B))
+
+    def tune(self, key, value):
+        _check(lib().pfs_tune_set(self._h, key.encode(), int(value)))
+
+    def timing(self, on):
+        _check(lib().pfs_timing_enable(self._h, 1 if on else 0))
+
+    def timing_collect(self):
+        """{kernel: (launches, total_ms)} for the launches recorded since the last call."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().pfs_timing_collect(self._h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().strip().splitlines():
+            name, cnt, tot = line.split(",")
+            out[name] = (int(cnt), float(tot))
+        return out
 
     # ---- host-buffer transforms (numpy complex128) ----------------------
     def dsoft(self, B, grid):

@@ -201,6 +201,8 @@ struct pfs_context {
     std::vector<hipEvent_t> ev_pool;
     struct Rec { const char* name; hipEvent_t a, b; };
     std::vector<Rec> recs;
+    // tuning knobs (pfs_tune_set); 0 = automatic
+    long fft_chunk_slices = 0;
 };
 
 namespace {
@@ -410,11 +412,36 @@ hipStream_t pick(pfs_context* c, void* s)
     return (hipStream_t)s;
 }
 
+// Slices per FFT chunk.  The row pass of a chunk writes W, the column pass
+// reads it right back: sized so W stays in the Infinity Cache (MALL) and the
+// stage streams the grid through HBM about once per direction.
+int fft_chunk(const pfs_context* c, int N)
+{
+    long want = c->fft_chunk_slices;
+    if (want <= 0) {
+        long per_slice = (long)N * N * (long)sizeof(double2);
+        want = (32L << 20) / per_slice;
+    }
+    int k = 8;
+    while (k * 2 <= want && k * 2 <= N) k *= 2;
+    if (k > N || N % k) k = N;
+    return k;
+}
+
 int fft_fwd(pfs_context* c, const Plan& p, const double2* grid, double2* W, double2* G, hipStream_t st)
 {
     const int N = p.t.N;
-    HIP_TRY(timed(c, "fft_fwd_rows", st, [&] { return launch_fft_lines_f64(grid, W, map_rows_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st); }));
-    HIP_TRY(timed(c, "fft_fwd_cols", st, [&] { return launch_fft_lines_f64(W, G, map_cols_fwd(N), N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st); }));
+    const int K = fft_chunk(c, N);
+    const long n2 = (long)N * N;
+    for (int k0 = 0; k0 < N; k0 += K) {
+        LineMap a = map_rows_fwd(N);
+        a.nu = K;
+        LineMap b = map_cols_fwd(N);
+        b.nv = K;
+        const double2* in = grid + k0 * n2;
+        HIP_TRY(timed(c, "fft_fwd_rows", st, [&] { return launch_fft_lines_f64(in, W, a, N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st); }));
+        HIP_TRY(timed(c, "fft_fwd_cols", st, [&] { return launch_fft_lines_f64(W, G + k0, b, N, -1, 1.0, p.t.tw, LINES_ROW_IN_TILE_OUT, 1, st); }));
+    }
     return PFS_OK;
 }
 
@@ -422,10 +449,19 @@ int fft_inv(pfs_context* c, const Plan& p, const double2* G, double2* W, double2
 {
     const int N = p.t.N;
     const int B = p.B;
-    HIP_TRY(timed(c, "fft_inv_cols", st, [&] { return launch_fft_lines_f64(G, W, map_cols_inv(N), N, +1, 1.0, p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st); }));
-    HIP_TRY(timed(c, "fft_inv_rows", st, [&] {
-        return launch_fft_lines_f64(W, grid, map_rows_inv(N), N, +1, 1. / (4. * B * B), p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st);
-    }));
+    const int K = fft_chunk(c, N);
+    const long n2 = (long)N * N;
+    for (int k0 = 0; k0 < N; k0 += K) {
+        LineMap b = map_cols_inv(N);
+        b.nv = K;
+        LineMap a = map_rows_inv(N);
+        a.nu = K;
+        double2* out = grid + k0 * n2;
+        HIP_TRY(timed(c, "fft_inv_cols", st, [&] { return launch_fft_lines_f64(G + k0, W, b, N, +1, 1.0, p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st); }));
+        HIP_TRY(timed(c, "fft_inv_rows", st, [&] {
+            return launch_fft_lines_f64(W, out, a, N, +1, 1. / (4. * B * B), p.t.tw, LINES_TILE_IN_ROW_OUT, 1, st);
+        }));
+    }
     return PFS_OK;
 }
 
@@ -475,12 +511,17 @@ int dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double sc
     if (rc) return rc;
     double2* a = (double2*)arr_dev;
     double2* W = (double2*)c->work[0];
-    LineMap m1 = map_rows_fwd(n);
-    m1.nu = lays;
-    LineMap m2 = map_cols_seam(n);
-    m2.nu = lays;
-    HIP_TRY(launch_fft_lines_f64(a, W, m1, n, sign, 1.0, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
-    HIP_TRY(launch_fft_lines_f64(W, a, m2, n, sign, scale, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    const int K = std::min(fft_chunk(c, n), lays);
+    const long n2 = (long)n * n;
+    for (int k0 = 0; k0 < lays; k0 += K) {
+        const int kc = std::min(K, lays - k0);
+        LineMap m1 = map_rows_fwd(n);
+        m1.nu = kc;
+        LineMap m2 = map_cols_seam(n);
+        m2.nu = kc;
+        HIP_TRY(launch_fft_lines_f64(a + k0 * n2, W, m1, n, sign, 1.0, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+        HIP_TRY(launch_fft_lines_f64(W, a + k0 * n2, m2, n, sign, scale, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+    }
     return PFS_OK;
 }
 
@@ -751,6 +792,29 @@ int pfs_stage_fft_inv_dev(pfs_context* c, int B, const void* pencils_dev, void* 
     rc = ensure_work(c, grid_bytes(B));
     if (rc) return rc;
     return fft_inv(c, *p, (const double2*)pencils_dev, (double2*)c->work[0], (double2*)grid_dev, pick(c, stream));
+}
+
+int pfs_tune_set(pfs_context* c, const char* key, long value)
+{
+    if (!c || !key) return fail(PFS_ERR_ARG, "pfs_tune_set: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!strcmp(key, "fft_chunk_slices")) {
+        if (value < 0) return fail(PFS_ERR_ARG, "fft_chunk_slices must be >= 0");
+        c->fft_chunk_slices = value;
+        return PFS_OK;
+    }
+    return fail(PFS_ERR_ARG, "pfs_tune_set: unknown key '%s'", key);
+}
+
+int pfs_tune_get(pfs_context* c, const char* key, long* value)
+{
+    if (!c || !key || !value) return fail(PFS_ERR_ARG, "pfs_tune_get: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!strcmp(key, "fft_chunk_slices")) {
+        *value = c->fft_chunk_slices;
+        return PFS_OK;
+    }
+    return fail(PFS_ERR_ARG, "pfs_tune_get: unknown key '%s'", key);
 }
 
 int pfs_timing_enable(pfs_context* c, int on)

@@ -259,6 +259,7 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
     constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;
     size_t smem = (size_t)LINES * (N + 1) * sizeof(typename CT<T>::C);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
+    if (m.nv % LINES != 0) return hipErrorInvalidValue;
     dim3 grid(m.nv / LINES, m.nu, nsig);
     hipLaunchKernelGGL((k_fft_lines<T, LOGN, LINES, KIND, SIGN>), grid, dim3(256), smem, st, in, out, m, tw, scale);
     return hipGetLastError();
@@ -287,7 +288,7 @@ template <typename T>
 static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m, int N,
                                int sign, T scale, const typename CT<T>::C* tw, int kind, int nsig, hipStream_t st)
 {
-    if (N < 2 || (N & 1) || m.nv != N || nsig < 1) return hipErrorInvalidValue;
+    if (N < 2 || (N & 1) || m.nv < 1 || m.nu < 1 || nsig < 1) return hipErrorInvalidValue;
     int logn = 0;
     while ((1 << logn) < N) ++logn;
     if ((1 << logn) == N) {
@@ -307,6 +308,7 @@ static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* o
         }
     if (!lines) return hipErrorInvalidValue;
     size_t smem = 2 * (size_t)lines * (N + 1) * sizeof(typename CT<T>::C);
+    if (m.nv % lines != 0) return hipErrorInvalidValue;
     dim3 grid(m.nv / lines, m.nu, nsig);
     if (kind == LINES_ROW_IN_TILE_OUT) {
         if (sign < 0)

@@ -218,3 +218,17 @@ def test_cpp_dropin_example_like_reference(ctx):
     assert r.returncode == 0
     assert "Correct result:  Yes" in r.stdout
     assert "[PFSOFTLib warning]" in r.stderr and "not even" in r.stderr
+
+
+@pytest.mark.parametrize("B", [64, 128])
+def test_fft_chunking_is_bitwise_invariant(ctx, torch, B):
+    # each line transform is the same whatever the slice chunking
+    c = O.seeded_coeffs(B, 31)
+    outs = []
+    for chunk in [8, 16, 2 * B, 0]:
+        ctx.tune("fft_chunk_slices", chunk)
+        g = ctx.idsoft(B, c)
+        outs.append((g, ctx.dsoft(B, g)))
+    ctx.tune("fft_chunk_slices", 0)
+    for g, r in outs[1:]:
+        assert np.array_equal(g, outs[0][0]) and np.array_equal(r, outs[0][1])
