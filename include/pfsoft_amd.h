@@ -130,9 +130,10 @@ int pfs_stage_dwt_inv_dev(pfs_context* ctx, int bandwidth, const void* coeffs_de
 int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_dev, void* grid_dev, void* stream);
 
 /* Tuning knobs (no counterpart in the reference).  Keys:
- *   "fft_chunk_slices"  beta-slices per FFT chunk (0 = automatic, ~32 MiB of
- *                       slices so the intermediate stays in the Infinity
- *                       Cache); rounded to a power of two dividing 2B. */
+ *   "fft_chunk_slices"  beta-slices per FFT chunk (0 = automatic: one chunk);
+ *                       rounded to a power of two dividing 2B.  Smaller
+ *                       chunks keep the row-pass output in the Infinity
+ *                       Cache at the price of more, smaller launches. */
 int pfs_tune_set(pfs_context* ctx, const char* key, long value);
 int pfs_tune_get(pfs_context* ctx, const char* key, long* value);
 

@@ -413,15 +413,13 @@ hipStream_t pick(pfs_context* c, void* s)
 }
 
 // Slices per FFT chunk.  The row pass of a chunk writes W, the column pass
-// reads it right back: sized so W stays in the Infinity Cache (MALL) and the
-// stage streams the grid through HBM about once per direction.
+// reads it right back, so a chunk of a few tens of MB keeps W in the
+// Infinity Cache (MALL).  Measured on MI355X at 2B = 256 the smaller launches
+// cost more than the cache saves, so the automatic choice is one chunk.
 int fft_chunk(const pfs_context* c, int N)
 {
     long want = c->fft_chunk_slices;
-    if (want <= 0) {
-        long per_slice = (long)N * N * (long)sizeof(double2);
-        want = (32L << 20) / per_slice;
-    }
+    if (want <= 0) return N;
     int k = 8;
     while (k * 2 <= want && k * 2 <= N) k *= 2;
     if (k > N || N % k) k = N;

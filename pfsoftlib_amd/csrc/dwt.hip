@@ -4,8 +4,8 @@
 // 2B >= 512).  Lane i of the wave runs the three-term recurrence of the
 // L2-normalised Wigner d-function (fn_dwt.hpp:152-199: base row, then rows
 // l = M+1 .. B-1) in FP64 registers for the beta nodes k = i, i+64, ... --
-// one recurrence chain per 64 nodes.  Sixteen degrees of one chain at a time
-// go through a 16 x 64 LDS tile (ping-pong buffered) into the
+// one recurrence chain per 64 nodes.  Sixteen degrees of all chains at a time
+// go through a 16 x (64 x chains) LDS tile into the
 // v_mfma_f64_16x16x4_f64 operand layout; the MFMAs contract them against the
 // 16 real right-hand sides of the pair.  The 8 symmetric cases
 // (l, +-M, +-M') / (l, +-M', +-M) of the reference (src/fn_dsoft.cpp:112-283,
@@ -34,8 +34,6 @@
 
 namespace pfs {
 
-constexpr int TSTR = 66;            // LDS tile row stride (doubles): 64 nodes + 2 pad
-constexpr int TILE = 16 * TSTR;     // one 16-degree x 64-node tile
 constexpr int OSTR = 18;            // output staging row stride (doubles), 64 rows
 
 struct CaseInfo {
@@ -136,31 +134,40 @@ __device__ __forceinline__ void init_chains(Chains<NCH>& ch, const PlanTables& t
     }
 }
 
-// Writes degrees r0 .. r0+15 of chain c into the tile (row rr, column lane)
-// and advances the chain past them.
+// Writes degrees r0 .. r0+15 of every chain into the tile (row rr, column
+// 64c + lane) and advances the chains past them.  The step coefficients are
+// read once per degree for all chains.
 template <int NCH>
-__device__ __forceinline__ void run_chain(Chains<NCH>& ch, int c, double* tile, int lane, const double* cf, int RP,
-                                          int r0)
+__device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int tstr, int lane, const double* cf,
+                                           int RP, int r0)
 {
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
-        int r = r0 + rr;
-        tile[rr * TSTR + lane] = ch.cur[c];
-        double u = fma(cf[r], ch.cb[c], cf[RP + r]);
-        double dn = fma(ch.cur[c], u, cf[2 * RP + r] * ch.prev[c]);
-        ch.prev[c] = ch.cur[c];
-        ch.cur[c] = dn;
+        const int r = r0 + rr;
+        const double f1 = cf[r], f1f2 = cf[RP + r], c1 = cf[2 * RP + r];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            tile[rr * tstr + 64 * c + lane] = ch.cur[c];
+            double u = fma(f1, ch.cb[c], f1f2);
+            double dn = fma(ch.cur[c], u, c1 * ch.prev[c]);
+            ch.prev[c] = ch.cur[c];
+            ch.cur[c] = dn;
+        }
     }
 }
 
+// LDS tile row stride for NCH chains: 64 NCH nodes + 2 pad (doubles).
+__host__ __device__ constexpr int tile_stride(int nch) { return 64 * nch + 2; }
+
 // Forward DWT.  NCH chains per lane (64 nodes each), WPP waves per pair
-// splitting the nodes; KQ = nodes per chain that carry data (64, or N
-// rounded up to 4 when N < 64).
-template <int NCH, int WPP>
+// splitting the nodes.  SMALL: 2B < 64, only the first KQ = 2B rounded up to
+// 4 nodes carry data.
+template <int NCH, int WPP, bool SMALL>
 __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,
                                                           double2* __restrict__ coeffs, double norm)
 {
     extern __shared__ __align__(16) double sm[];
+    constexpr int TS = tile_stride(NCH);
     const int B = t.B, N = t.N;
     const int2 pr = t.pairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
@@ -168,18 +175,17 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
     const int nch = (R + 15) >> 4;
     const int RP = nch * 16;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int KQ = N >= 64 ? 64 : ((N + 3) & ~3);
-    const int nmf = KQ >> 2;  // MFMAs per chain and chunk
+    const int nmf = SMALL ? ((N + 3) >> 2) : 16;  // MFMAs per chain and chunk
 
-    double* tiles = sm + w * 2 * TILE;
-    double* red = sm + WPP * 2 * TILE;   // WPP * 256 (only when WPP > 1)
+    double* tile = sm + w * 16 * TS;
+    double* red = sm + WPP * 16 * TS;   // WPP * 256 (only when WPP > 1)
     double* cf = red + (WPP > 1 ? WPP * 256 : 0);
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
 
-    const int kbase = w * 64 * NCH + lane;
+    const int kw = w * 64 * NCH;
     Chains<NCH> ch;
-    init_chains<NCH>(ch, t, M, Mp, kbase, true);
+    init_chains<NCH>(ch, t, M, Mp, kw + lane, true);
 
     // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
     const int j = lane & 15, kk = lane >> 4;
@@ -188,32 +194,30 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
     double sreg[NCH * 16];
 #pragma unroll
     for (int m = 0; m < NCH * 16; ++m) {
-        int ks = w * 64 * NCH + 4 * m + kk;
+        int ks = kw + 4 * m + kk;
         double v = 0.0;
-        if (ci.active && ks < N && ((4 * m) & 63) < KQ) {
+        if (ci.active && ks < N) {
             int kr = ci.flip ? N - 1 - ks : ks;
             v = vsrc[2 * kr];
         }
         sreg[m] = v;
     }
-    __syncthreads();
 
     for (int chk = 0; chk < nch; ++chk) {
+        __syncthreads();  // coefficient table ready / previous chunk's tile reads done
+        run_chains<NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
+        __syncthreads();
         v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            double* tile = tiles + (c & 1) * TILE;
-            run_chain<NCH>(ch, c, tile, lane, cf, RP, chk * 16);
-            __syncthreads();
+        for (int c = 0; c < NCH; ++c)
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
-                if (m < nmf) {
-                    double a = tile[(lane & 15) * TSTR + 4 * m + (lane >> 4)];
+                if (!SMALL || m < nmf) {
+                    double a = tile[(lane & 15) * TS + 64 * c + 4 * m + (lane >> 4)];
                     if (m & 1) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[16 * c + m], acc1, 0, 0, 0);
                     else acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[16 * c + m], acc0, 0, 0, 0);
                 }
             }
-        }
         v4d acc = acc0 + acc1;
         if (WPP > 1) {
 #pragma unroll
@@ -227,7 +231,6 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
                     acc[v] = s;
                 }
             }
-            __syncthreads();
         }
         if (w == 0) {
             const int jj = mfma_out_col(lane);
@@ -248,11 +251,12 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
 }
 
 // Inverse DWT: every wave owns its nodes outright (no reduction).
-template <int NCH, int WPP>
+template <int NCH, int WPP, bool SMALL>
 __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const double2* __restrict__ coeffs,
                                                           double2* __restrict__ G, double norm)
 {
     extern __shared__ __align__(16) double sm[];
+    constexpr int TS = tile_stride(NCH);
     const int B = t.B, N = t.N;
     const int2 pr = t.pairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
@@ -260,11 +264,10 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     const int nch = (R + 15) >> 4;
     const int RP = nch * 16;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int KQ = N >= 64 ? 64 : ((N + 3) & ~3);
-    const int nkb = KQ >= 16 ? 4 : 1;  // 16-node blocks per chain
+    const int nkb = SMALL ? ((N + 15) >> 4) : 4;  // 16-node blocks per chain
 
-    double* tiles = sm + w * 2 * TILE;
-    double* cf = sm + WPP * 2 * TILE;
+    double* tile = sm + w * 16 * TS;
+    double* cf = sm + WPP * 16 * TS;
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
 
@@ -293,36 +296,34 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
     double bcur[4], bnext[4];
     load_b(0, bcur);
-    __syncthreads();
 
     for (int chk = 0; chk < nch; ++chk) {
         if (chk + 1 < nch) load_b(chk + 1, bnext);
+        __syncthreads();
+        run_chains<NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
+        __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            double* tile = tiles + (c & 1) * TILE;
-            run_chain<NCH>(ch, c, tile, lane, cf, RP, chk * 16);
-            __syncthreads();
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int c = 0; c < NCH; ++c)
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb)
-                    if (kb < nkb) {
-                        double a = tile[(4 * s + bsel) * TSTR + 16 * kb + j];
+                    if (!SMALL || kb < nkb) {
+                        double a = tile[(4 * s + bsel) * TS + 64 * c + 16 * kb + j];
                         acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[s], acc[4 * c + kb], 0, 0, 0);
                     }
-        }
 #pragma unroll
         for (int s = 0; s < 4; ++s) bcur[s] = bnext[s];
     }
 
     // stage each chain's 64 x 16 block and write each case's beta-vector run
-    double* stage = tiles;  // 64 * OSTR <= 2 * TILE
+    double* stage = tile;  // 64 * OSTR <= 16 * TS
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         __syncthreads();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
-            if (kb < nkb)
+            if (!SMALL || kb < nkb)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                     stage[(16 * kb + mfma_out_row(lane, v)) * OSTR + mfma_out_col(lane)] = acc[4 * c + kb][v];
@@ -345,61 +346,57 @@ struct DwtShape {
     int nch, wpp;
 };
 
-// Forward: up to 4 chains per lane in one wave (the cross-wave reduction is
-// paid only above 256 nodes).  Inverse: 2 chains per lane and one wave per
-// 128 nodes -- its waves need no reduction and fewer accumulators keep more
-// waves resident.
-static DwtShape dwt_shape(int N, bool fwd)
+// NCH chains of 64 nodes per lane and WPP waves per pair: one wave covers up
+// to 256 nodes; larger grids split the nodes over waves (the forward sums the
+// waves' partial blocks through LDS, the inverse needs no reduction).
+static DwtShape dwt_shape(int N)
 {
     if (N <= 64) return {1, 1};
     if (N <= 128) return {2, 1};
-    if (fwd) {
-        if (N <= 256) return {4, 1};
-        if (N <= 512) return {4, 2};
-        return {4, 4};
-    }
-    if (N <= 256) return {2, 2};
-    if (N <= 512) return {2, 4};
-    return {2, 8};
+    if (N <= 256) return {4, 1};
+    if (N <= 512) return {4, 2};
+    return {4, 4};
 }
 
 static size_t dwt_smem(int B, int N, bool fwd)
 {
-    DwtShape s = dwt_shape(N, fwd);
+    DwtShape s = dwt_shape(N);
     int RP = ((B + 15) / 16) * 16;
-    size_t d = (size_t)s.wpp * 2 * TILE + 3 * RP;
+    size_t d = (size_t)s.wpp * 16 * tile_stride(s.nch) + 3 * RP;
     if (fwd && s.wpp > 1) d += (size_t)s.wpp * 256;
     return d * sizeof(double);
 }
 
-#define PFS_DWT_LAUNCH(KERNEL, NCH, WPP, ...) \
-    hipLaunchKernelGGL((KERNEL<NCH, WPP>), dim3(t.npairs), dim3(64 * WPP), smem, st, __VA_ARGS__)
+#define PFS_DWT_LAUNCH(KERNEL, NCH, WPP, SMALL, ...) \
+    hipLaunchKernelGGL((KERNEL<NCH, WPP, SMALL>), dim3(t.npairs), dim3(64 * WPP), smem, st, __VA_ARGS__)
 
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
-    DwtShape s = dwt_shape(t.N, true);
+    DwtShape s = dwt_shape(t.N);
     size_t smem = dwt_smem(t.B, t.N, true);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 1, 1, t, G, coeffs, norm);
-    else if (s.nch == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 2, 1, t, G, coeffs, norm);
-    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 1, t, G, coeffs, norm);
-    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 2, t, G, coeffs, norm);
-    else PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 4, t, G, coeffs, norm);
+    if (t.N < 64) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 1, 1, true, t, G, coeffs, norm);
+    else if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 1, 1, false, t, G, coeffs, norm);
+    else if (s.nch == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 2, 1, false, t, G, coeffs, norm);
+    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 1, false, t, G, coeffs, norm);
+    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 2, false, t, G, coeffs, norm);
+    else PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 4, false, t, G, coeffs, norm);
     return hipGetLastError();
 }
 
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
-    DwtShape s = dwt_shape(t.N, false);
+    DwtShape s = dwt_shape(t.N);
     size_t smem = dwt_smem(t.B, t.N, false);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 1, 1, t, coeffs, G, norm);
-    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 1, t, coeffs, G, norm);
-    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 2, t, coeffs, G, norm);
-    else if (s.wpp == 4) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 4, t, coeffs, G, norm);
-    else PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 8, t, coeffs, G, norm);
+    if (t.N < 64) PFS_DWT_LAUNCH(k_dwt_inv_f64, 1, 1, true, t, coeffs, G, norm);
+    else if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 1, 1, false, t, coeffs, G, norm);
+    else if (s.nch == 2) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 1, false, t, coeffs, G, norm);
+    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 4, 1, false, t, coeffs, G, norm);
+    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_inv_f64, 4, 2, false, t, coeffs, G, norm);
+    else PFS_DWT_LAUNCH(k_dwt_inv_f64, 4, 4, false, t, coeffs, G, norm);
     return hipGetLastError();
 }
 

@@ -251,11 +251,151 @@ __global__ __launch_bounds__(256) void k_dft_lines(const typename CT<T>::C* __re
     store_tile<KIND>(b, ls, out, ob, m.out_sv, m.out_sn, N, lines, scale);
 }
 
+// ---------------------------------------------------------------------------
+// Register four-step line FFT for N = 64, 128, 256 (TPL = N/16 threads per
+// line, 16 complex values per thread).  With n = n1 + TPL n2 and
+// k = k2 + 16 k1:
+//   step A  (thread n1)  Y[n1][k2] = sum_n2 x[n1 + TPL n2] W_16^(n2 k2)   -- read from HBM
+//           Y[n1][k2] *= W_N^(n1 k2)
+//   LDS transpose
+//   step B  (thread t)   X[k2 + 16 k1] = sum_n1 Y[n1][k2] W_TPL^(n1 k1),  k2 = t + TPL q  -- written to HBM
+// One LDS exchange per line instead of one per radix pass.
+
+template <int SIGN, typename C> __device__ __forceinline__ C tw16(C a, int e)
+{
+    // a * exp(SIGN 2 pi i e / 16), e a compile-time constant after unrolling
+    typedef decltype(a.x) R;
+    const R c1 = (R)0.92387953251128675613, s1 = (R)0.38268343236508977173, h = (R)0.70710678118654752440;
+    R c, s;
+    switch (e & 15) {
+    case 0: return a;
+    case 1: c = c1; s = s1; break;
+    case 2: c = h; s = h; break;
+    case 3: c = s1; s = c1; break;
+    case 4: c = 0; s = 1; break;
+    case 5: c = -s1; s = c1; break;
+    case 6: c = -h; s = h; break;
+    case 7: c = -c1; s = s1; break;
+    case 8: c = -1; s = 0; break;
+    case 9: c = -c1; s = -s1; break;
+    case 10: c = -h; s = -h; break;
+    case 11: c = -s1; s = -c1; break;
+    case 12: c = 0; s = -1; break;
+    case 13: c = s1; s = -c1; break;
+    case 14: c = h; s = -h; break;
+    default: c = c1; s = -s1; break;
+    }
+    C w;
+    w.x = c;
+    w.y = SIGN * s;
+    return cmul(a, w);
+}
+
+// 16-point DFT in registers; on return X[k] sits in v[(k >> 2) + 4 (k & 3)].
+template <int SIGN, typename C> __device__ __forceinline__ void dft16_perm(C* v)
+{
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) dft4<SIGN>(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]);
+    // v[n1 + 4 k2] = Y[n1][k2]; twiddle W16^(n1 k2)
+#pragma unroll
+    for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+        for (int k2 = 1; k2 < 4; ++k2) v[n1 + 4 * k2] = tw16<SIGN>(v[n1 + 4 * k2], n1 * k2);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) dft4<SIGN>(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+}
+
+template <int Rn> __device__ __forceinline__ constexpr int perm_index(int k)
+{
+    return Rn == 16 ? ((k >> 2) + 4 * (k & 3)) : k;
+}
+
+template <int Rn, int SIGN, typename C> __device__ __forceinline__ void dft_small(C* v)
+{
+    if (Rn == 16) dft16_perm<SIGN>(v);
+    else if (Rn == 8) dft8<SIGN>(v);
+    else if (Rn == 4) dft4<SIGN>(v[0], v[1], v[2], v[3]);
+    else dft2<SIGN>(v);
+}
+
+template <typename T, int LOGN, int KIND, int SIGN>
+__global__ __launch_bounds__(8 * (1 << LOGN) / 16) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
+                                                                          typename CT<T>::C* __restrict__ out,
+                                                                          LineMap m,
+                                                                          const typename CT<T>::C* __restrict__ tw,
+                                                                          T scale)
+{
+    typedef typename CT<T>::C C;
+    constexpr int N = 1 << LOGN;
+    constexpr int TPL = N / 16;          // threads per line; also the step-B DFT size
+    constexpr int L = 8;                 // lines per workgroup
+    constexpr int YS = TPL + 1;          // LDS row stride (complex) for one k2
+    constexpr int LS = 16 * YS + 1;      // LDS stride between lines
+    __shared__ C lds[L * LS];
+
+    const int tid = threadIdx.x;
+    const long u = blockIdx.y, v0 = (long)blockIdx.x * L, sig = blockIdx.z;
+    const C* src = in + sig * m.in_sig + u * m.in_su + v0 * m.in_sv;
+    C* dst = out + sig * m.out_sig + u * m.out_su + v0 * m.out_sv;
+
+    // step A: rows read with n1 fastest, tiles with the line fastest
+    int la, n1;
+    if (KIND == LINES_ROW_IN_TILE_OUT) { n1 = tid % TPL; la = tid / TPL; }
+    else { la = tid % L; n1 = tid / L; }
+    C v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[la * m.in_sv + (long)(n1 + TPL * n2) * m.in_sn];
+    dft16_perm<SIGN>(v);
+    C* ly = lds + la * LS;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+        C y = v[perm_index<16>(k2)];
+        const int e = n1 * k2;  // < N
+        if (e) {
+            C w = tw[e];
+            if (SIGN > 0) w.y = -w.y;
+            y = cmul(y, w);
+        }
+        ly[k2 * YS + n1] = y;
+    }
+    __syncthreads();
+
+    // step B: 16 / TPL DFTs of size TPL per thread; row stores with k2 fastest,
+    // tile stores with the line fastest
+    int lb, t;
+    if (KIND == LINES_ROW_IN_TILE_OUT) { lb = tid % L; t = tid / L; }
+    else { t = tid % TPL; lb = tid / TPL; }
+    const C* lyb = lds + lb * LS;
+    constexpr int Q = 16 / TPL;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int k2 = t + TPL * q;
+        C x[TPL];
+#pragma unroll
+        for (int a = 0; a < TPL; ++a) x[a] = lyb[k2 * YS + a];
+        dft_small<TPL, SIGN>(x);
+#pragma unroll
+        for (int k1 = 0; k1 < TPL; ++k1) {
+            C r = x[perm_index<TPL>(k1)];
+            r.x *= scale;
+            r.y *= scale;
+            dst[lb * m.out_sv + (long)(k2 + 16 * k1) * m.out_sn] = r;
+        }
+    }
+}
+
 template <typename T, int LOGN, int KIND, int SIGN>
 static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m,
                               const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st)
 {
     constexpr int N = 1 << LOGN;
+    if (LOGN >= 6 && LOGN <= 8) {
+        if (m.nv % 8 != 0) return hipErrorInvalidValue;
+        dim3 grid(m.nv / 8, m.nu, nsig);
+        hipLaunchKernelGGL((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
+                           dim3(8 * N / 16), 0, st, in, out, m, tw, scale);
+        return hipGetLastError();
+    }
     constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;
     size_t smem = (size_t)LINES * (N + 1) * sizeof(typename CT<T>::C);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
