@@ -134,10 +134,10 @@ __device__ __forceinline__ void init_chains(Chains<NCH>& ch, const PlanTables& t
     }
 }
 
-// Writes degrees r0 .. r0+15 of every chain into the tile (row rr, column
-// 64c + lane) and advances the chains past them.  The step coefficients are
-// read once per degree for all chains.
-template <int NCH>
+// Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
+// column 64 (c - C0) + lane) and advances them.  The step coefficients are
+// read once per degree for these chains.
+template <int NCH, int C0, int NC>
 __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int tstr, int lane, const double* cf,
                                            int RP, int r0)
 {
@@ -146,8 +146,8 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int ts
         const int r = r0 + rr;
         const double f1 = cf[r], f1f2 = cf[RP + r], c1 = cf[2 * RP + r];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            tile[rr * tstr + 64 * c + lane] = ch.cur[c];
+        for (int c = C0; c < C0 + NC; ++c) {
+            tile[rr * tstr + 64 * (c - C0) + lane] = ch.cur[c];
             double u = fma(f1, ch.cb[c], f1f2);
             double dn = fma(ch.cur[c], u, c1 * ch.prev[c]);
             ch.prev[c] = ch.cur[c];
@@ -156,18 +156,40 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int ts
     }
 }
 
-// LDS tile row stride for NCH chains: 64 NCH nodes + 2 pad (doubles).
-__host__ __device__ constexpr int tile_stride(int nch) { return 64 * nch + 2; }
+// LDS tile row stride for a tile of nc chains: 64 nc nodes + 1 (odd, so the
+// 16 rows an MFMA operand read touches fall in distinct banks).
+__host__ __device__ constexpr int tile_stride(int nc) { return 64 * nc + 1; }
 
-// Forward DWT.  NCH chains per lane (64 nodes each), WPP waves per pair
-// splitting the nodes.  SMALL: 2B < 64, only the first KQ = 2B rounded up to
-// 4 nodes carry data.
-template <int NCH, int WPP, bool SMALL>
+// Per-lane view of one case of the pair (hoisted out of the degree loop).
+struct LaneCase {
+    int Mc, Mpc, active;
+    double sgn;      // sign for even rows
+    double sgn_odd;  // sign for odd rows (differs for the reversed cases)
+};
+
+__device__ __forceinline__ LaneCase lane_case(int c, int M, int Mp, int N, long* vec)
+{
+    CaseInfo ci = case_info(c, M, Mp, N);
+    LaneCase lc;
+    lc.Mc = ci.Mc;
+    lc.Mpc = ci.Mpc;
+    lc.active = ci.active;
+    lc.sgn = ci.sgn;
+    lc.sgn_odd = ci.flip ? -ci.sgn : ci.sgn;
+    if (vec) *vec = ci.vec;
+    return lc;
+}
+
+// Forward DWT.  NCH chains of 64 nodes per lane, processed NH at a time
+// through a 16 x 64 NH tile; WPP waves per pair split the nodes and sum
+// their partial blocks through LDS.  SMALL: 2B < 64, only the first 2B
+// nodes (rounded up to 4) carry data.
+template <int NCH, int NH, int WPP, bool SMALL>
 __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,
                                                           double2* __restrict__ coeffs, double norm)
 {
     extern __shared__ __align__(16) double sm[];
-    constexpr int TS = tile_stride(NCH);
+    constexpr int TS = tile_stride(NH);
     const int B = t.B, N = t.N;
     const int2 pr = t.pairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
@@ -189,61 +211,64 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
 
     // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
     const int j = lane & 15, kk = lane >> 4;
-    const CaseInfo ci = case_info(j >> 1, M, Mp, N);
-    const double* vsrc = reinterpret_cast<const double*>(G + ci.vec) + (j & 1);
+    long vec;
+    const LaneCase lc = lane_case(j >> 1, M, Mp, N, &vec);
+    const bool flip = lc.sgn != lc.sgn_odd;
+    const double* vsrc = reinterpret_cast<const double*>(G + vec) + (j & 1);
     double sreg[NCH * 16];
 #pragma unroll
     for (int m = 0; m < NCH * 16; ++m) {
         int ks = kw + 4 * m + kk;
         double v = 0.0;
-        if (ci.active && ks < N) {
-            int kr = ci.flip ? N - 1 - ks : ks;
-            v = vsrc[2 * kr];
-        }
+        if (lc.active && ks < N) v = vsrc[2 * (flip ? N - 1 - ks : ks)];
         sreg[m] = v;
     }
+    // output rows of this lane: 4v + lane/16 within each chunk
+    const int orow = lane >> 4;
 
     for (int chk = 0; chk < nch; ++chk) {
-        __syncthreads();  // coefficient table ready / previous chunk's tile reads done
-        run_chains<NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
-        __syncthreads();
         v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
+        for (int h = 0; h < NCH / NH; ++h) {
+            __syncthreads();  // coefficient table ready / previous tile reads done
+            if (h == 0) run_chains<NCH, 0, NH>(ch, tile, TS, lane, cf, RP, chk * 16);
+            else if (h == 1) run_chains<NCH, (NH < NCH ? NH : 0), NH>(ch, tile, TS, lane, cf, RP, chk * 16);
+            else if (h == 2) run_chains<NCH, (2 * NH < NCH ? 2 * NH : 0), NH>(ch, tile, TS, lane, cf, RP, chk * 16);
+            else run_chains<NCH, (3 * NH < NCH ? 3 * NH : 0), NH>(ch, tile, TS, lane, cf, RP, chk * 16);
+            __syncthreads();
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                if (!SMALL || m < nmf) {
-                    double a = tile[(lane & 15) * TS + 64 * c + 4 * m + (lane >> 4)];
-                    if (m & 1) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[16 * c + m], acc1, 0, 0, 0);
-                    else acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[16 * c + m], acc0, 0, 0, 0);
+            for (int c = 0; c < NH; ++c)
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    if (!SMALL || m < nmf) {
+                        double a = tile[(lane & 15) * TS + 64 * c + 4 * m + kk];
+                        double b = sreg[16 * (h * NH + c) + m];
+                        if (m & 1) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc1, 0, 0, 0);
+                        else acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
+                    }
                 }
-            }
+        }
         v4d acc = acc0 + acc1;
         if (WPP > 1) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) red[w * 256 + mfma_out_row(lane, v) * 16 + mfma_out_col(lane)] = acc[v];
+            for (int v = 0; v < 4; ++v) red[w * 256 + (4 * v + orow) * 16 + j] = acc[v];
             __syncthreads();
             if (w == 0) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     double s = 0.0;
-                    for (int ww = 0; ww < WPP; ++ww) s += red[ww * 256 + mfma_out_row(lane, v) * 16 + mfma_out_col(lane)];
+                    for (int ww = 0; ww < WPP; ++ww) s += red[ww * 256 + (4 * v + orow) * 16 + j];
                     acc[v] = s;
                 }
             }
         }
-        if (w == 0) {
-            const int jj = mfma_out_col(lane);
-            const CaseInfo co = case_info(jj >> 1, M, Mp, N);
-            if (co.active) {
+        if (w == 0 && lc.active) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    int r = chk * 16 + mfma_out_row(lane, v);
-                    if (r < R) {
-                        double sg = (co.flip && (r & 1)) ? -co.sgn : co.sgn;
-                        reinterpret_cast<double*>(coeffs + coeff_index(M + r, co.Mc, co.Mpc))[jj & 1] =
-                            norm * (sg * acc[v]);
-                    }
+            for (int v = 0; v < 4; ++v) {
+                const int r = chk * 16 + 4 * v + orow;
+                if (r < R) {
+                    const double sg = (r & 1) ? lc.sgn_odd : lc.sgn;
+                    reinterpret_cast<double*>(coeffs + coeff_index32(M + r, lc.Mc, lc.Mpc))[j & 1] = norm * (sg * acc[v]);
                 }
             }
         }
@@ -277,15 +302,16 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
 
     // B operand rows: lane (j + 16 b) holds n' * sign * coeff(l = M + r0 + 4s + b, case j/2).(re|im)
     const int j = lane & 15, bsel = lane >> 4;
-    const CaseInfo ci = case_info(j >> 1, M, Mp, N);
+    const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
+    const double* csrc = reinterpret_cast<const double*>(coeffs) + (j & 1);
     auto load_b = [&](int chk, double* b) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            int r = chk * 16 + 4 * s + bsel;
+            const int r = chk * 16 + 4 * s + bsel;
             double v = 0.0;
-            if (ci.active && r < R) {
-                double sg = (ci.flip && (r & 1)) ? -ci.sgn : ci.sgn;
-                v = sg * (norm * reinterpret_cast<const double*>(coeffs + coeff_index(M + r, ci.Mc, ci.Mpc))[j & 1]);
+            if (lc.active && r < R) {
+                const double sg = (r & 1) ? lc.sgn_odd : lc.sgn;
+                v = sg * (norm * csrc[2 * coeff_index32(M + r, lc.Mc, lc.Mpc)]);
             }
             b[s] = v;
         }
@@ -300,7 +326,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     for (int chk = 0; chk < nch; ++chk) {
         if (chk + 1 < nch) load_b(chk + 1, bnext);
         __syncthreads();
-        run_chains<NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
+        run_chains<NCH, 0, NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -343,60 +369,66 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
 }
 
 struct DwtShape {
-    int nch, wpp;
+    int nch, nh, wpp;
 };
 
-// NCH chains of 64 nodes per lane and WPP waves per pair: one wave covers up
-// to 256 nodes; larger grids split the nodes over waves (the forward sums the
-// waves' partial blocks through LDS, the inverse needs no reduction).
-static DwtShape dwt_shape(int N)
+// Forward: one wave per pair up to 256 nodes, chains two at a time through
+// the tile (16 KB) so two waves fit per SIMD; above 256 nodes the nodes are
+// split over waves.  Inverse: two chains per wave and one wave per 128 nodes
+// (its waves need no reduction; fewer accumulators keep occupancy up).
+static DwtShape dwt_shape(int N, bool fwd)
 {
-    if (N <= 64) return {1, 1};
-    if (N <= 128) return {2, 1};
-    if (N <= 256) return {4, 1};
-    if (N <= 512) return {4, 2};
-    return {4, 4};
+    if (N <= 64) return {1, 1, 1};
+    if (fwd) {
+        if (N <= 128) return {2, 2, 1};
+        if (N <= 256) return {4, 2, 1};
+        if (N <= 512) return {4, 2, 2};
+        return {4, 2, 4};
+    }
+    if (N <= 128) return {2, 2, 1};
+    if (N <= 256) return {2, 2, 2};
+    if (N <= 512) return {2, 2, 4};
+    return {2, 2, 8};
 }
 
 static size_t dwt_smem(int B, int N, bool fwd)
 {
-    DwtShape s = dwt_shape(N);
+    DwtShape s = dwt_shape(N, fwd);
     int RP = ((B + 15) / 16) * 16;
-    size_t d = (size_t)s.wpp * 16 * tile_stride(s.nch) + 3 * RP;
+    size_t d = (size_t)s.wpp * 16 * tile_stride(s.nh) + 3 * RP;
     if (fwd && s.wpp > 1) d += (size_t)s.wpp * 256;
     return d * sizeof(double);
 }
 
-#define PFS_DWT_LAUNCH(KERNEL, NCH, WPP, SMALL, ...) \
-    hipLaunchKernelGGL((KERNEL<NCH, WPP, SMALL>), dim3(t.npairs), dim3(64 * WPP), smem, st, __VA_ARGS__)
-
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
-    DwtShape s = dwt_shape(t.N);
+    DwtShape s = dwt_shape(t.N, true);
     size_t smem = dwt_smem(t.B, t.N, true);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    if (t.N < 64) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 1, 1, true, t, G, coeffs, norm);
-    else if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 1, 1, false, t, G, coeffs, norm);
-    else if (s.nch == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 2, 1, false, t, G, coeffs, norm);
-    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 1, false, t, G, coeffs, norm);
-    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 2, false, t, G, coeffs, norm);
-    else PFS_DWT_LAUNCH(k_dwt_fwd_f64, 4, 4, false, t, G, coeffs, norm);
+    dim3 grid(t.npairs), block(64 * s.wpp);
+    if (t.N < 64) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.nch == 2) hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.wpp == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.wpp == 2) hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 2, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 4, false>), grid, block, smem, st, t, G, coeffs, norm);
     return hipGetLastError();
 }
 
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
-    DwtShape s = dwt_shape(t.N);
+    DwtShape s = dwt_shape(t.N, false);
     size_t smem = dwt_smem(t.B, t.N, false);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    if (t.N < 64) PFS_DWT_LAUNCH(k_dwt_inv_f64, 1, 1, true, t, coeffs, G, norm);
-    else if (s.nch == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 1, 1, false, t, coeffs, G, norm);
-    else if (s.nch == 2) PFS_DWT_LAUNCH(k_dwt_inv_f64, 2, 1, false, t, coeffs, G, norm);
-    else if (s.wpp == 1) PFS_DWT_LAUNCH(k_dwt_inv_f64, 4, 1, false, t, coeffs, G, norm);
-    else if (s.wpp == 2) PFS_DWT_LAUNCH(k_dwt_inv_f64, 4, 2, false, t, coeffs, G, norm);
-    else PFS_DWT_LAUNCH(k_dwt_inv_f64, 4, 4, false, t, coeffs, G, norm);
+    dim3 grid(t.npairs), block(64 * s.wpp);
+    if (t.N < 64) hipLaunchKernelGGL((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_inv_f64<1, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.wpp == 1) hipLaunchKernelGGL((k_dwt_inv_f64<2, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.wpp == 2) hipLaunchKernelGGL((k_dwt_inv_f64<2, 2, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.wpp == 4) hipLaunchKernelGGL((k_dwt_inv_f64<2, 4, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else hipLaunchKernelGGL((k_dwt_inv_f64<2, 8, false>), grid, block, smem, st, t, coeffs, G, norm);
     return hipGetLastError();
 }
 

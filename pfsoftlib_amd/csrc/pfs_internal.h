@@ -30,6 +30,15 @@ __host__ __device__ inline long coeff_index(int l, int M, int Mp)
     return coeff_offset(l) + iM + n * iMp;
 }
 
+// 32-bit form for device index math: exact for l < 512 (4 l^3 < 2^31).
+__device__ __forceinline__ int coeff_index32(int l, int M, int Mp)
+{
+    int n = 2 * l + 1;
+    int iM = M >= 0 ? M : n + M;
+    int iMp = Mp >= 0 ? Mp : n + Mp;
+    return (4 * l * l * l - l) / 3 + iM + n * iMp;
+}
+
 // Device-resident tables for one bandwidth (all FP64 unless noted).
 struct PlanTables {
     int B, N;
