@@ -117,34 +117,60 @@ struct Chains {
     double cb[NCH], cur[NCH], prev[NCH];
 };
 
+// Loads the node tables of every chain; separate from init_chains so the
+// loads can all be in flight before any of them is used.
 template <int NCH>
-__device__ __forceinline__ void init_chains(Chains<NCH>& ch, const PlanTables& t, int M, int Mp, int kbase,
-                                            bool weighted)
+struct ChainTables {
+    double cb[NCH], sh[NCH], chh[NCH], w[NCH];
+};
+
+template <int NCH>
+__device__ __forceinline__ void load_chain_tables(ChainTables<NCH>& ct, const PlanTables& t, int kbase, bool weighted)
 {
-    const double nf = t.normfac[M * t.B + Mp];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        int k = kbase + 64 * c;
-        bool kv = k < t.N;
-        ch.cb[c] = kv ? t.cosb[k] : 0.0;
-        double b = kv ? nf * ipow(t.sinh[k], M - Mp) * ipow(t.cosh[k], M + Mp) : 0.0;
-        if (weighted && kv) b *= t.wts[k];
-        ch.cur[c] = b;
+        const int k = min(kbase + 64 * c, t.N - 1);
+        ct.cb[c] = t.cosb[k];
+        ct.sh[c] = t.sinh[k];
+        ct.chh[c] = t.cosh[k];
+        ct.w[c] = weighted ? t.wts[k] : 1.0;
+    }
+}
+
+template <int NCH>
+__device__ __forceinline__ void init_chains(Chains<NCH>& ch, const ChainTables<NCH>& ct, double nf, int M, int Mp,
+                                            int kbase, int N)
+{
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const bool kv = kbase + 64 * c < N;
+        ch.cb[c] = kv ? ct.cb[c] : 0.0;
+        ch.cur[c] = kv ? nf * ipow(ct.sh[c], M - Mp) * ipow(ct.chh[c], M + Mp) * ct.w[c] : 0.0;
         ch.prev[c] = 0.0;
     }
 }
 
-// Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
-// column 64 (c - C0) + lane) and advances them.  The step coefficients are
-// read once per degree for these chains.
-template <int NCH, int C0, int NC>
-__device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int tstr, int lane, const double* cf,
-                                           int RP, int r0)
+// Broadcast lane `src` (uniform) of a double to every lane, via SGPRs.
+__device__ __forceinline__ double bcast_lane(double v, int src)
 {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
+// Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
+// column 64 (c - C0) + lane) and advances them.  Lanes 0..15 fetch the step
+// coefficients of the 16 degrees in one LDS read each; every step then takes
+// them from SGPRs, so no LDS round trip sits on the recurrence chain.
+template <int NCH, int C0, int NC>
+__device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__ tile, int tstr, int lane,
+                                           const double* __restrict__ cf, int RP, int r0)
+{
+    const int rl = r0 + (lane & 15);
+    const double vf1 = cf[rl], vf1f2 = cf[RP + rl], vc1 = cf[2 * RP + rl];
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
-        const int r = r0 + rr;
-        const double f1 = cf[r], f1f2 = cf[RP + r], c1 = cf[2 * RP + r];
+        const double f1 = bcast_lane(vf1, rr), f1f2 = bcast_lane(vf1f2, rr), c1 = bcast_lane(vc1, rr);
 #pragma unroll
         for (int c = C0; c < C0 + NC; ++c) {
             tile[rr * tstr + 64 * (c - C0) + lane] = ch.cur[c];
@@ -185,7 +211,7 @@ __device__ __forceinline__ LaneCase lane_case(int c, int M, int Mp, int N, long*
 // their partial blocks through LDS.  SMALL: 2B < 64, only the first 2B
 // nodes (rounded up to 4) carry data.
 template <int NCH, int NH, int WPP, bool SMALL>
-__global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,
+__global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) void k_dwt_fwd_f64(PlanTables t, const double2* __restrict__ G,
                                                           double2* __restrict__ coeffs, double norm)
 {
     extern __shared__ __align__(16) double sm[];
@@ -203,26 +229,32 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
     double* red = sm + WPP * 16 * TS;   // WPP * 256 (only when WPP > 1)
     double* cf = red + (WPP > 1 ? WPP * 256 : 0);
 
-    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
-
     const int kw = w * 64 * NCH;
-    Chains<NCH> ch;
-    init_chains<NCH>(ch, t, M, Mp, kw + lane, true);
 
-    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
+    // issue every global load of the preamble before any of them is used
     const int j = lane & 15, kk = lane >> 4;
     long vec;
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, &vec);
     const bool flip = lc.sgn != lc.sgn_odd;
     const double* vsrc = reinterpret_cast<const double*>(G + vec) + (j & 1);
+    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
     double sreg[NCH * 16];
 #pragma unroll
     for (int m = 0; m < NCH * 16; ++m) {
-        int ks = kw + 4 * m + kk;
-        double v = 0.0;
-        if (lc.active && ks < N) v = vsrc[2 * (flip ? N - 1 - ks : ks)];
-        sreg[m] = v;
+        const int ks = kw + 4 * m + kk;
+        const int kr = min(flip ? N - 1 - ks : ks, N - 1);
+        sreg[m] = vsrc[2 * max(kr, 0)];
     }
+    ChainTables<NCH> ct;
+    load_chain_tables<NCH>(ct, t, kw + lane, true);
+    const double nf = t.normfac[M * B + Mp];
+
+    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+#pragma unroll
+    for (int m = 0; m < NCH * 16; ++m)
+        if (!lc.active || kw + 4 * m + kk >= N) sreg[m] = 0.0;
     // output rows of this lane: 4v + lane/16 within each chunk
     const int orow = lane >> 4;
 
@@ -277,7 +309,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
 
 // Inverse DWT: every wave owns its nodes outright (no reduction).
 template <int NCH, int WPP, bool SMALL>
-__global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const double2* __restrict__ coeffs,
+__global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) void k_dwt_inv_f64(PlanTables t, const double2* __restrict__ coeffs,
                                                           double2* __restrict__ G, double norm)
 {
     extern __shared__ __align__(16) double sm[];
@@ -294,11 +326,10 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     double* tile = sm + w * 16 * TS;
     double* cf = sm + WPP * 16 * TS;
 
-    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
-
     const int kw = w * 64 * NCH;
-    Chains<NCH> ch;
-    init_chains<NCH>(ch, t, M, Mp, kw + lane, false);
+    ChainTables<NCH> ct;
+    load_chain_tables<NCH>(ct, t, kw + lane, false);
+    const double nf = t.normfac[M * B + Mp];
 
     // B operand rows: lane (j + 16 b) holds n' * sign * coeff(l = M + r0 + 4s + b, case j/2).(re|im)
     const int j = lane & 15, bsel = lane >> 4;
@@ -316,12 +347,16 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
             b[s] = v;
         }
     };
+    double bcur[4], bnext[4];
+    load_b(0, bcur);
+
+    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
 
     v4d acc[NCH * 4];
 #pragma unroll
     for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
-    double bcur[4], bnext[4];
-    load_b(0, bcur);
 
     for (int chk = 0; chk < nch; ++chk) {
         if (chk + 1 < nch) load_b(chk + 1, bnext);
@@ -343,6 +378,15 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     }
 
     // stage each chain's 64 x 16 block and write each case's beta-vector run
+    long cvec[8];
+    int cflip = 0, cact = 0;
+#pragma unroll
+    for (int cs = 0; cs < 8; ++cs) {
+        const CaseInfo co = case_info(cs, M, Mp, N);
+        cvec[cs] = co.vec;
+        cflip |= co.flip << cs;
+        cact |= co.active << cs;
+    }
     double* stage = tile;  // 64 * OSTR <= 16 * TS
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
@@ -356,13 +400,13 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
         __syncthreads();
         const int k = kw + 64 * c + lane;
         if (k < N) {
+#pragma unroll
             for (int cs = 0; cs < 8; ++cs) {
-                CaseInfo co = case_info(cs, M, Mp, N);
-                if (!co.active) continue;
+                if (!((cact >> cs) & 1)) continue;
                 double2 val;
                 val.x = stage[lane * OSTR + 2 * cs];
                 val.y = stage[lane * OSTR + 2 * cs + 1];
-                G[co.vec + (co.flip ? N - 1 - k : k)] = val;
+                G[cvec[cs] + (((cflip >> cs) & 1) ? N - 1 - k : k)] = val;
             }
         }
     }
@@ -373,8 +417,8 @@ struct DwtShape {
 };
 
 // Forward: one wave per pair up to 256 nodes, chains two at a time through
-// the tile (16 KB) so two waves fit per SIMD; above 256 nodes the nodes are
-// split over waves.  Inverse: two chains per wave and one wave per 128 nodes
+// the tile (16 KB) so two waves fit per SIMD; above 256 nodes one wave per
+// 128 nodes, their partial blocks summed through LDS.  Inverse: two chains per wave and one wave per 128 nodes
 // (its waves need no reduction; fewer accumulators keep occupancy up).
 static DwtShape dwt_shape(int N, bool fwd)
 {
@@ -382,8 +426,8 @@ static DwtShape dwt_shape(int N, bool fwd)
     if (fwd) {
         if (N <= 128) return {2, 2, 1};
         if (N <= 256) return {4, 2, 1};
-        if (N <= 512) return {4, 2, 2};
-        return {4, 2, 4};
+        if (N <= 512) return {2, 2, 4};
+        return {2, 2, 8};
     }
     if (N <= 128) return {2, 2, 1};
     if (N <= 256) return {2, 2, 2};
@@ -411,8 +455,8 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
     else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
     else if (s.nch == 2) hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
     else if (s.wpp == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.wpp == 2) hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 2, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 4, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.wpp == 4) hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 4, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 8, false>), grid, block, smem, st, t, G, coeffs, norm);
     return hipGetLastError();
 }
 
