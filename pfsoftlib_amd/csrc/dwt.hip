@@ -1,7 +1,7 @@
 // Discrete Wigner transform stage, FP64, Wigner-d generated on the fly.
 //
-// One wave per canonical pair (M, M'), M >= M' >= 0 (two or four waves for
-// 2B >= 512).  Lane i of the wave runs the three-term recurrence of the
+// One wave per canonical pair (M, M'), M >= M' >= 0, for 2B <= 256 nodes
+// (dwt_large.hip takes larger grids).  Lane i of the wave runs the three-term recurrence of the
 // L2-normalised Wigner d-function (fn_dwt.hpp:152-199: base row, then rows
 // l = M+1 .. B-1) in FP64 registers for the beta nodes k = i, i+64, ... --
 // one recurrence chain per 64 nodes.  Sixteen degrees of all chains at a time
@@ -416,23 +416,15 @@ struct DwtShape {
     int nch, nh, wpp;
 };
 
-// Forward: one wave per pair up to 256 nodes, chains two at a time through
-// the tile (16 KB) so two waves fit per SIMD; above 256 nodes one wave per
-// 128 nodes, their partial blocks summed through LDS.  Inverse: two chains per wave and one wave per 128 nodes
-// (its waves need no reduction; fewer accumulators keep occupancy up).
+// Grids up to 256 nodes (larger ones go to dwt_large.hip).  Forward: one wave
+// per pair, chains two at a time through a 16 KB tile so two waves fit per
+// SIMD.  Inverse: two chains per lane and one wave per 128 nodes (its waves
+// need no reduction; fewer accumulators keep occupancy up).
 static DwtShape dwt_shape(int N, bool fwd)
 {
     if (N <= 64) return {1, 1, 1};
-    if (fwd) {
-        if (N <= 128) return {2, 2, 1};
-        if (N <= 256) return {4, 2, 1};
-        if (N <= 512) return {2, 2, 4};
-        return {2, 2, 8};
-    }
     if (N <= 128) return {2, 2, 1};
-    if (N <= 256) return {2, 2, 2};
-    if (N <= 512) return {2, 2, 4};
-    return {2, 2, 8};
+    return fwd ? DwtShape{4, 2, 1} : DwtShape{2, 2, 2};
 }
 
 static size_t dwt_smem(int B, int N, bool fwd)
@@ -447,6 +439,7 @@ static size_t dwt_smem(int B, int N, bool fwd)
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
+    if (t.N > 256) return launch_dwt_fwd_f64_large(t, G, coeffs, norm, st);
     DwtShape s = dwt_shape(t.N, true);
     size_t smem = dwt_smem(t.B, t.N, true);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
@@ -454,15 +447,14 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
     if (t.N < 64) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
     else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
     else if (s.nch == 2) hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.wpp == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.wpp == 4) hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 4, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 8, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
     return hipGetLastError();
 }
 
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
+    if (t.N > 256) return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
     DwtShape s = dwt_shape(t.N, false);
     size_t smem = dwt_smem(t.B, t.N, false);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
@@ -470,9 +462,7 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     if (t.N < 64) hipLaunchKernelGGL((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm);
     else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_inv_f64<1, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
     else if (s.wpp == 1) hipLaunchKernelGGL((k_dwt_inv_f64<2, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.wpp == 2) hipLaunchKernelGGL((k_dwt_inv_f64<2, 2, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.wpp == 4) hipLaunchKernelGGL((k_dwt_inv_f64<2, 4, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else hipLaunchKernelGGL((k_dwt_inv_f64<2, 8, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else hipLaunchKernelGGL((k_dwt_inv_f64<2, 2, false>), grid, block, smem, st, t, coeffs, G, norm);
     return hipGetLastError();
 }
 

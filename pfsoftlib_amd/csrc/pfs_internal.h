@@ -83,6 +83,12 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm,
                               hipStream_t st);
 
+// Variants for 2B >= 512 (dwt_large.hip).
+hipError_t launch_dwt_fwd_f64_large(const PlanTables& t, const double2* G, double2* coeffs, double norm,
+                                    hipStream_t st);
+hipError_t launch_dwt_inv_f64_large(const PlanTables& t, const double2* coeffs, double2* G, double norm,
+                                    hipStream_t st);
+
 // Copy the grid entries the inverse DWT never writes (row == B or col == B)
 // into the pencil buffer, so the inverse FFT reads the caller's values there.
 hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st);
