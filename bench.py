@@ -177,6 +177,14 @@ def main():
         "roundtrip_rel_l2": err,
         "kernels": kernels,
     }
+    # PCIe-inclusive rate of the host-buffer entry points (not the headline value)
+    c_host = coeffs.cpu().numpy()
+    g_host = ctx.idsoft(B, c_host)
+    t2 = time.perf_counter()
+    for _ in range(3):
+        g_host = ctx.idsoft(B, c_host)
+        ctx.dsoft(B, g_host)
+    out["host_buffer_pairs_per_s"] = 3 / (time.perf_counter() - t2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
     print(json.dumps(out))
