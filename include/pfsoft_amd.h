@@ -92,6 +92,19 @@ int pfs_idsoft_f64(pfs_context* ctx, int bandwidth, const double* coeffs, double
 int pfs_dsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* grid_dev, void* coeffs_dev, void* stream);
 int pfs_idsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* grid_dev, void* stream);
 
+/* Batched mixed-precision transforms of nsig signals of one bandwidth
+ * (no counterpart in the reference, which transforms one complex128 grid per
+ * call).  grids: nsig consecutive N^3 complex64 grids (reference layout);
+ * coeffs: nsig consecutive packed complex64 coefficient sets.  Slice FFTs and
+ * the Wigner contraction run in FP32 (MFMA); Wigner tables, quadrature weights
+ * and twiddles are FP64-computed.  The inverse writes every grid entry; those
+ * with row or column index B (never produced by the Wigner stage) are 0.
+ * Supported B: 8, 16, 32, 64, 128. */
+int pfs_dsoft_batch_f32_dev(pfs_context* ctx, int bandwidth, int nsig, const void* grids_dev, void* coeffs_dev,
+                            void* stream);
+int pfs_idsoft_batch_f32_dev(pfs_context* ctx, int bandwidth, int nsig, const void* coeffs_dev, void* grids_dev,
+                             void* stream);
+
 /* Layer-wise unnormalised 2D DFT of lays square slices, in place, host
  * memory.  Replaces uzl_fftw_layer_wise_DFT2_grid3D / _IDFT2_grid3D
  * (include/PFSOFTlib_headers/fftw_wrapper.hpp:28-33, src/fftw_wrapper.cpp:47-172):
@@ -133,7 +146,9 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *   "fft_chunk_slices"  beta-slices per FFT chunk (0 = automatic: one chunk);
  *                       rounded to a power of two dividing 2B.  Smaller
  *                       chunks keep the row-pass output in the Infinity
- *                       Cache at the price of more, smaller launches. */
+ *                       Cache at the price of more, smaller launches.
+ *   "batch_chunk_signals" signals per sub-batch of the batched path (work
+ *                       buffers hold this many complex64 grids; default 64). */
 int pfs_tune_set(pfs_context* ctx, const char* key, long value);
 int pfs_tune_get(pfs_context* ctx, const char* key, long* value);
 

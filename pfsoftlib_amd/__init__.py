@@ -64,6 +64,8 @@ def lib():
             "pfs_stage_dwt_fwd_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_stage_dwt_inv_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_stage_fft_inv_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_dsoft_batch_f32_dev": (ip, [vp, ip, ip, vp, vp, vp]),
+            "pfs_idsoft_batch_f32_dev": (ip, [vp, ip, ip, vp, vp, vp]),
             "pfs_tune_set": (ip, [vp, ctypes.c_char_p, ctypes.c_long]),
             "pfs_tune_get": (ip, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]),
             "pfs_timing_enable": (ip, [vp, ip]),
@@ -188,6 +190,15 @@ class Context:
 
     def idsoft_dev(self, B, coeffs_dev, grid_dev, stream=None):
         _check(lib().pfs_idsoft_f64_dev(self._h, B, _devptr(coeffs_dev), _devptr(grid_dev), _stream_handle(stream)))
+
+    def dsoft_batch_f32(self, B, nsig, grids_dev, coeffs_dev, stream=None):
+        """nsig complex64 grids -> nsig packed complex64 coefficient sets (mixed precision)."""
+        _check(lib().pfs_dsoft_batch_f32_dev(self._h, B, nsig, _devptr(grids_dev), _devptr(coeffs_dev),
+                                             _stream_handle(stream)))
+
+    def idsoft_batch_f32(self, B, nsig, coeffs_dev, grids_dev, stream=None):
+        _check(lib().pfs_idsoft_batch_f32_dev(self._h, B, nsig, _devptr(coeffs_dev), _devptr(grids_dev),
+                                              _stream_handle(stream)))
 
     def stage_fft_fwd(self, B, grid_dev, pencils_dev, stream=None):
         _check(lib().pfs_stage_fft_fwd_dev(self._h, B, _devptr(grid_dev), _devptr(pencils_dev), _stream_handle(stream)))

@@ -141,6 +141,53 @@ __global__ void k_mfma_probe(double* out)
     }
 }
 
+// FP32 16x16x4 probe: A[i][ksel] = i + 1, B[ksel][j] = 2^(6j); the product
+// (i+1) 2^(6j) is exact in FP32 and names row and column.
+__global__ void k_mfma_probe_f32(float* out)
+{
+    const int lane = threadIdx.x;
+    for (int t = 0; t < 2; ++t) {
+        const int ksel = t == 0 ? 0 : 3;
+        float a = (lane >> 4) == ksel ? (float)((lane & 15) + 1) : 0.0f;
+        float b = (lane >> 4) == ksel ? ldexpf(1.0f, 6 * (lane & 15)) : 0.0f;
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+        for (int v = 0; v < 4; ++v) out[t * 256 + lane * 4 + v] = acc[v];
+    }
+}
+
+int mfma_selftest_f32(hipStream_t st)
+{
+    float* d = nullptr;
+    HIP_TRY(hipMalloc(&d, 512 * sizeof(float)));
+    hipLaunchKernelGGL(k_mfma_probe_f32, dim3(1), dim3(64), 0, st, d);
+    float h[512];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(PFS_ERR_DEVICE, "MFMA f32 self-test: %s", hipGetErrorString(e));
+    bool ok[2] = {true, true};  // layout 0: row 4v + L/16; layout 1: row 4(L/16) + v
+    for (int t = 0; t < 2; ++t)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int v = 0; v < 4; ++v) {
+                float x = h[t * 256 + lane * 4 + v];
+                int ex = 0;
+                frexpf(x, &ex);
+                int jj = (ex - 1) / 6;
+                float im1 = ldexpf(x, -6 * jj);
+                int i = (int)im1 - 1;
+                bool exact = x > 0 && im1 == (float)(i + 1) && i >= 0 && i < 16;
+                int col = lane & 15;
+                if (!(exact && jj == col && i == 4 * v + (lane >> 4))) ok[0] = false;
+                if (!(exact && jj == col && i == 4 * (lane >> 4) + v)) ok[1] = false;
+            }
+    if (ok[0]) g_mfma_f32_layout = 0;
+    else if (ok[1]) g_mfma_f32_layout = 1;
+    else return fail(PFS_ERR_UNSUPPORTED, "MFMA f32 16x16x4 output layout matches neither known form");
+    return PFS_OK;
+}
+
 int mfma_selftest(hipStream_t st)
 {
     double* d = nullptr;
@@ -174,6 +221,11 @@ struct Plan {
     PlanTables t{};
     void* mem = nullptr;  // one allocation for all tables
     size_t bytes = 0;
+    // FP32 Wigner tables of the batched path (built on first use)
+    void* d32 = nullptr;  // row offsets (long, per pair) followed by the table
+    size_t d32_bytes = 0;
+    const long* roff = nullptr;
+    const float* tab32 = nullptr;
 };
 
 }  // namespace
@@ -203,6 +255,10 @@ struct pfs_context {
     std::vector<Rec> recs;
     // tuning knobs (pfs_tune_set); 0 = automatic
     long fft_chunk_slices = 0;
+    long batch_chunk_signals = 64;
+    // batched-path work buffers (complex64, batch_chunk_signals grids each)
+    void* bwork[2] = {nullptr, nullptr};
+    size_t bwork_bytes = 0;
 };
 
 namespace {
@@ -523,6 +579,113 @@ int dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double sc
     return PFS_OK;
 }
 
+// ---- batched mixed-precision pipeline ---------------------------------------
+
+int ensure_table32(pfs_context* c, Plan& p, hipStream_t st)
+{
+    if (p.d32) return PFS_OK;
+    const int B = p.B, N = p.t.N;
+    std::vector<int2> pairs;
+    for (int M = 0; M < B; ++M)
+        for (int Mp = 0; Mp <= M; ++Mp) pairs.push_back(make_int2(M, Mp));
+    std::stable_sort(pairs.begin(), pairs.end(), [](int2 a, int2 b) { return a.x < b.x; });
+    std::vector<long> roff(pairs.size());
+    long rows = 0;
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        roff[i] = rows;
+        rows += ((B - pairs[i].x + 15) / 16) * 16;
+    }
+    const size_t head = ((pairs.size() * sizeof(long)) + 255) & ~(size_t)255;
+    const size_t bytes = head + (size_t)rows * N * sizeof(float);
+    int rc = ctx_alloc(c, &p.d32, bytes);
+    if (rc) return rc;
+    p.d32_bytes = bytes;
+    p.roff = (const long*)p.d32;
+    p.tab32 = (const float*)((char*)p.d32 + head);
+    HIP_TRY(hipMemcpyAsync((void*)p.roff, roff.data(), pairs.size() * sizeof(long), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_wigner_table_f32(p.t, p.roff, (float*)p.tab32, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return PFS_OK;
+}
+
+int ensure_bwork(pfs_context* c, size_t bytes)
+{
+    if (c->bwork_bytes >= bytes) return PFS_OK;
+    (void)hipDeviceSynchronize();
+    ctx_free(c, c->bwork[0], c->bwork_bytes);
+    ctx_free(c, c->bwork[1], c->bwork_bytes);
+    c->bwork_bytes = 0;
+    int rc = ctx_alloc(c, &c->bwork[0], bytes);
+    if (rc) return rc;
+    rc = ctx_alloc(c, &c->bwork[1], bytes);
+    if (rc) {
+        ctx_free(c, c->bwork[0], bytes);
+        return rc;
+    }
+    c->bwork_bytes = bytes;
+    return PFS_OK;
+}
+
+int batch_prepare(pfs_context* c, int B, int nsig, Plan** pp, int* sb, hipStream_t st)
+{
+    if (nsig < 1) return fail(PFS_ERR_ARG, "batched transform: nsig must be >= 1");
+    int rc = get_plan(c, B, pp);
+    if (rc) return rc;
+    if (!batch_supported((*pp)->t.N))
+        return fail(PFS_ERR_UNSUPPORTED, "batched mixed-precision path supports B = 8, 16, 32, 64, 128 (got %d)", B);
+    rc = ensure_table32(c, **pp, st);
+    if (rc) return rc;
+    *sb = (int)std::min<long>(c->batch_chunk_signals, nsig);
+    const size_t N = 2 * (size_t)B;
+    return ensure_bwork(c, (size_t)*sb * N * N * N * sizeof(float2));
+}
+
+int dsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* grids, float2* coeffs, hipStream_t st)
+{
+    Plan* p;
+    int sb;
+    int rc = batch_prepare(c, B, nsig, &p, &sb, st);
+    if (rc) return rc;
+    const int N = p->t.N;
+    const long N3 = (long)N * N * N, count = coeff_offset(B);
+    float2* W = (float2*)c->bwork[0];
+    float2* G = (float2*)c->bwork[1];
+    LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
+    a.in_sig = a.out_sig = b.in_sig = b.out_sig = N3;
+    const float norm = (float)fwd_norm(B);
+    for (int s0 = 0; s0 < nsig; s0 += sb) {
+        const int n = std::min(sb, nsig - s0);
+        HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(grids + s0 * N3, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
+        HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, G, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->roff, p->tab32, G, coeffs + s0 * count, n, norm, st); }));
+    }
+    return PFS_OK;
+}
+
+int idsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* coeffs, float2* grids, hipStream_t st)
+{
+    Plan* p;
+    int sb;
+    int rc = batch_prepare(c, B, nsig, &p, &sb, st);
+    if (rc) return rc;
+    const int N = p->t.N;
+    const long N3 = (long)N * N * N, count = coeff_offset(B);
+    float2* W = (float2*)c->bwork[0];
+    float2* G = (float2*)c->bwork[1];
+    LineMap b = map_cols_inv(N), a = map_rows_inv(N);
+    a.in_sig = a.out_sig = b.in_sig = b.out_sig = N3;
+    const float norm = (float)inv_norm(B);
+    const float scale = (float)(1. / (4. * B * B));
+    for (int s0 = 0; s0 < nsig; s0 += sb) {
+        const int n = std::min(sb, nsig - s0);
+        HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(G, B, n, st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->roff, p->tab32, coeffs + s0 * count, G, n, norm, st); }));
+        HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
+        HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids + s0 * N3, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
+    }
+    return PFS_OK;
+}
+
 std::mutex g_seam_mu;
 pfs_context* g_seam_ctx = nullptr;
 
@@ -579,6 +742,7 @@ int pfs_ctx_create(int device, size_t hbm_budget_bytes, pfs_context** out)
         return fail(PFS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     int rc = mfma_selftest(c->stream);
+    if (!rc) rc = mfma_selftest_f32(c->stream);
     if (rc) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -595,7 +759,12 @@ int pfs_ctx_destroy(pfs_context* c)
         std::lock_guard<std::mutex> g(c->mu);
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
-        for (auto& kv : c->plans) ctx_free(c, kv.second.mem, kv.second.bytes);
+        for (auto& kv : c->plans) {
+            ctx_free(c, kv.second.mem, kv.second.bytes);
+            ctx_free(c, kv.second.d32, kv.second.d32_bytes);
+        }
+        ctx_free(c, c->bwork[0], c->bwork_bytes);
+        ctx_free(c, c->bwork[1], c->bwork_bytes);
         c->plans.clear();
         ctx_free(c, c->work[0], c->work_bytes);
         ctx_free(c, c->work[1], c->work_bytes);
@@ -792,6 +961,22 @@ int pfs_stage_fft_inv_dev(pfs_context* c, int B, const void* pencils_dev, void* 
     return fft_inv(c, *p, (const double2*)pencils_dev, (double2*)c->work[0], (double2*)grid_dev, pick(c, stream));
 }
 
+int pfs_dsoft_batch_f32_dev(pfs_context* c, int B, int nsig, const void* grids_dev, void* coeffs_dev, void* stream)
+{
+    if (!c || !grids_dev || !coeffs_dev) return fail(PFS_ERR_ARG, "pfs_dsoft_batch_f32_dev: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return dsoft_batch_f32(c, B, nsig, (const float2*)grids_dev, (float2*)coeffs_dev, pick(c, stream));
+}
+
+int pfs_idsoft_batch_f32_dev(pfs_context* c, int B, int nsig, const void* coeffs_dev, void* grids_dev, void* stream)
+{
+    if (!c || !grids_dev || !coeffs_dev) return fail(PFS_ERR_ARG, "pfs_idsoft_batch_f32_dev: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return idsoft_batch_f32(c, B, nsig, (const float2*)coeffs_dev, (float2*)grids_dev, pick(c, stream));
+}
+
 int pfs_tune_set(pfs_context* c, const char* key, long value)
 {
     if (!c || !key) return fail(PFS_ERR_ARG, "pfs_tune_set: null argument");
@@ -799,6 +984,11 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
     if (!strcmp(key, "fft_chunk_slices")) {
         if (value < 0) return fail(PFS_ERR_ARG, "fft_chunk_slices must be >= 0");
         c->fft_chunk_slices = value;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_chunk_signals")) {
+        if (value < 1) return fail(PFS_ERR_ARG, "batch_chunk_signals must be >= 1");
+        c->batch_chunk_signals = value;
         return PFS_OK;
     }
     return fail(PFS_ERR_ARG, "pfs_tune_set: unknown key '%s'", key);
@@ -810,6 +1000,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     std::lock_guard<std::mutex> g(c->mu);
     if (!strcmp(key, "fft_chunk_slices")) {
         *value = c->fft_chunk_slices;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_chunk_signals")) {
+        *value = c->batch_chunk_signals;
         return PFS_OK;
     }
     return fail(PFS_ERR_ARG, "pfs_tune_get: unknown key '%s'", key);

@@ -53,6 +53,10 @@ struct PlanTables {
     const float2* tw32;    // same, rounded to FP32
 };
 
+// Output lane layout of v_mfma_f32_16x16x4f32 found by the context self-test:
+// 0: lane L holds D[4v + L/16][L%16]; 1: D[4(L/16) + v][L%16].
+extern int g_mfma_f32_layout;
+
 // Largest dynamic LDS allocation a workgroup may request on the current device.
 size_t lds_limit();
 
@@ -88,6 +92,15 @@ hipError_t launch_dwt_fwd_f64_large(const PlanTables& t, const double2* G, doubl
                                     hipStream_t st);
 hipError_t launch_dwt_inv_f64_large(const PlanTables& t, const double2* coeffs, double2* G, double norm,
                                     hipStream_t st);
+
+// ---- batched mixed-precision path (batch.hip) ------------------------------
+bool batch_supported(int N);
+hipError_t launch_wigner_table_f32(const PlanTables& t, const long* roff, float* D, hipStream_t st);
+hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const long* roff, const float* D, const float2* G,
+                                    float2* coeffs, int nsig, float norm, hipStream_t st);
+hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const long* roff, const float* D, const float2* coeffs,
+                                    float2* G, int nsig, float norm, hipStream_t st);
+hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st);
 
 // Copy the grid entries the inverse DWT never writes (row == B or col == B)
 // into the pencil buffer, so the inverse FFT reads the caller's values there.
