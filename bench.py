@@ -22,6 +22,7 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+FP32_MFMA_PEAK_TFS = 157.3  # MI355X dense FP32 matrix, published
 
 
 def algorithmic_bytes_per_step(kernel, B, e=16):
@@ -72,6 +73,95 @@ def cpu_baseline(B, pairs, seed):
     }
 
 
+def batch_dwt_flops(B):
+    """Useful FP32 flops of one signal's batched DWT in one direction: every
+    active (pair, case) contracts R = B - M degrees against N = 2B betas for
+    the real and the imaginary part."""
+    N = 2 * B
+    f = 0
+    for M in range(B):
+        for Mp in range(M + 1):
+            nc = 1 if M == 0 else (4 if (Mp == 0 or Mp == M) else 8)
+            f += nc * 2 * (B - M) * N * 2
+    return f
+
+
+def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed):
+    """The batched mixed-precision round trip over nsig complex64 signals.
+
+    Every signal's result is checked against the plain FP64 GPU path (which
+    the parity tests pin to the CPU oracle at <= 1e-11) on the same input:
+    inverse grids against FP64 iDSOFT of the complex64 coefficients, forward
+    coefficients against FP64 DSOFT of the complex64 grid."""
+    N = 2 * B
+    n = P.coeff_count(B)
+    stream = torch.cuda.current_stream()
+    c32 = P.seeded_coeffs_torch(B, [seed + i for i in range(nsig)], "cuda", dtype=torch.complex64)
+    grids = torch.empty((nsig, N ** 3), dtype=torch.complex64, device="cuda")
+    back = torch.empty_like(c32)
+    ctx.warm(B)
+
+    def step():
+        ctx.idsoft_batch_f32(B, nsig, c32, grids, stream=stream)
+        ctx.dsoft_batch_f32(B, nsig, grids, back, stream=stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(stream)
+        step()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.timing(False)
+    lat = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(steps))
+
+    kernels = {}
+    dwt_flops = batch_dwt_flops(B) * nsig
+    for name, (cnt, tot_ms) in ctx.timing_collect().items():
+        k = {"launches_per_step": cnt // steps, "us_per_step": round(tot_ms / steps * 1e3, 2),
+             "avg_launch_us": round(tot_ms / cnt * 1e3, 2)}
+        if name.startswith("batch_dwt_"):
+            tf = dwt_flops / (tot_ms / steps / 1e3) / 1e12
+            k["useful_TFLOPs"] = round(tf, 2)
+            k["frac_fp32_mfma"] = round(tf / FP32_MFMA_PEAK_TFS, 4)
+        if name.startswith("batch_fft_"):
+            gbs = nsig * N ** 3 * 8 * 2 / (tot_ms / steps / 1e3) / 1e9
+            k["achieved_GBps"] = round(gbs, 1)
+            k["frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
+        kernels[name] = k
+
+    # error of every signal against the FP64 path on the same input
+    g64 = torch.empty(N ** 3, dtype=torch.complex128, device="cuda")
+    c64 = torch.empty(n, dtype=torch.complex128, device="cuda")
+    e_inv, e_fwd, e_rt = [], [], []
+    nrm = torch.linalg.vector_norm
+    for i in range(nsig):
+        g64.zero_()  # iDSOFT reads the Nyquist entries of the grid it is given
+        ctx.idsoft_dev(B, c32[i].to(torch.complex128), g64, stream=stream)
+        e_inv.append((nrm(grids[i].to(torch.complex128) - g64) / nrm(g64)).item())
+        ctx.dsoft_dev(B, grids[i].to(torch.complex128), c64, stream=stream)
+        e_fwd.append((nrm(back[i].to(torch.complex128) - c64) / nrm(c64)).item())
+        ref = c32[i].to(torch.complex128)
+        e_rt.append((nrm(back[i].to(torch.complex128) - ref) / nrm(ref)).item())
+    elapsed = t1 - t0
+    return {
+        "bandwidth": B, "signals": nsig, "precision": "fp32 FFT + fp32 MFMA DWT, fp64 tables/weights/twiddles",
+        "pairs_per_s": nsig * steps / elapsed, "transforms_per_s": 2 * nsig * steps / elapsed,
+        "ms_per_batch_roundtrip": elapsed / steps * 1e3, "steps": steps, "warmup": warmup,
+        "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "rel_l2_vs_fp64": {"inverse_max": max(e_inv), "forward_max": max(e_fwd), "roundtrip_max": max(e_rt),
+                           "inverse_median": sorted(e_inv)[nsig // 2], "forward_median": sorted(e_fwd)[nsig // 2],
+                           "signals_checked": nsig, "bound": 1e-6},
+        "kernels": kernels,
+    }
+
+
 def copy_calibration(torch, steps, mib):
     """A plain elementwise copy of known size, for scaling PMC byte counters."""
     n = mib * (1 << 20) // 8
@@ -94,6 +184,10 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--fft-chunk", type=int, default=0, help="beta-slices per FFT chunk (0 = automatic)")
+    ap.add_argument("--batch-signals", type=int, default=64,
+                    help="signals per batched mixed-precision round trip at the same bandwidth (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--c4", action="store_true", help="also run 512 signals at B=64 through the batched path")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -185,6 +279,14 @@ def main():
         g_host = ctx.idsoft(B, c_host)
         ctx.dsoft(B, g_host)
     out["host_buffer_pairs_per_s"] = 3 / (time.perf_counter() - t2)
+    del grid, back
+    if args.batch_signals:
+        bl = batched_leg(torch, P, ctx, B, args.batch_signals, args.batch_steps, 1, args.seed)
+        out["batched_mixed"] = bl
+        out["batched_vs_plain_fp64"] = bl["pairs_per_s"] / out["value"]
+        torch.cuda.empty_cache()
+    if args.c4:
+        out["batched_mixed_512xB64"] = batched_leg(torch, P, ctx, 64, 512, args.batch_steps, 1, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
     print(json.dumps(out))

@@ -18,7 +18,7 @@ import sys
 
 import numpy as np
 
-from .layout import coeff_count, coeff_index, coeff_offset, seeded_coeffs  # noqa: F401
+from .layout import coeff_count, coeff_index, coeff_offset, seeded_coeffs, seeded_coeffs_torch  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBPATH = os.path.join(_HERE, "libpfsoft_amd.so")

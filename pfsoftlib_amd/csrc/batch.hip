@@ -6,12 +6,16 @@
 // FFTs and the DWT contraction run in FP32 (v_mfma_f32_16x16x4_f32, FP32
 // accumulation).
 //
-// DWT mapping: one workgroup per (canonical pair, group of signals).  For
-// every 16-degree chunk the pair's FP32 table rows are staged once in LDS
-// and shared by the workgroup's waves; each wave contracts them against its
-// own signals, whose 8 case beta-vectors (forward) it keeps staged in LDS.
-// The 8 symmetric cases, reversed-beta reads and signs follow the table in
-// dwt.hip (reference: src/fn_dsoft.cpp:112-283, src/fn_idsoft.cpp:100-283).
+// DWT mapping: one workgroup per (canonical pair, group of signals); each
+// wave owns SPW signals and computes, per signal, the pair's whole
+// (degrees x 16) forward product or (betas x 16) inverse product in MFMA
+// accumulators -- 16 columns = 8 symmetric cases x (re, im).  Table operands
+// come from 16 x 16 tiles laid out so that every lane loads a float4 and a
+// wave loads 1 KB contiguous; the workgroup's waves and the pair's other
+// signal groups (same XCD) reuse them from cache.  Forward signal operands
+// are read straight from the pencils, no LDS.  The 8 symmetric cases,
+// reversed-beta reads and signs follow the table in dwt.hip (reference:
+// src/fn_dsoft.cpp:112-283, src/fn_idsoft.cpp:100-283).
 #include "pfs_internal.h"
 
 namespace pfs {
@@ -56,22 +60,32 @@ __device__ __forceinline__ int out_row(int lane, int v)
     return LAYOUT == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v;
 }
 
-// FP32 table: for pair index pi, rows r < RP(pi) (16-padded, zero beyond R)
-// at D[(roff[pi] + r) * N + k], unweighted, positive base.  FP64 recurrence.
-__global__ void k_wigner_table_f32(PlanTables t, const long* __restrict__ roff, float* __restrict__ D)
+// FP32 tables for pair index pi, rows r < RP(pi) (16-padded, zero beyond R),
+// unweighted, positive base, from the FP64 recurrence.  Both tables are cut
+// into 16 x 16 tiles; tile (q, kb) of pair pi (rows 16q.., betas 16kb..) sits
+// at tile index (roff[pi]/16 + q) * N/16 + kb.  Tf stores a tile row-major
+// (r, k) -> 16 r + k, Ti column-major (r, k) -> 16 k + r, so that a lane of
+// either contraction reads its four operands as one float4.  Block 0 also
+// writes the FP32 quadrature weights.
+__global__ void k_wigner_tiles_f32(PlanTables t, const long* __restrict__ roff, float* __restrict__ wts32,
+                                   float* __restrict__ Tf, float* __restrict__ Ti)
 {
-    const int N = t.N, B = t.B;
+    const int N = t.N, B = t.B, NKB = N >> 4;
     const int2 pr = t.pairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y, R = B - M, RP = ((R + 15) >> 4) << 4;
-    float* out = D + roff[blockIdx.x] * N;
+    const long tile0 = (roff[blockIdx.x] >> 4) * NKB;
     const double nf = t.normfac[M * B + Mp];
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        if (blockIdx.x == 0) wts32[k] = (float)t.wts[k];
         double d = nf, dp = 0.0;
         const double sh = t.sinh[k], ch = t.cosh[k], cb = t.cosb[k];
         for (int e = 0; e < M - Mp; ++e) d *= sh;
         for (int e = 0; e < M + Mp; ++e) d *= ch;
         for (int r = 0; r < RP; ++r) {
-            out[(long)r * N + k] = r < R ? (float)d : 0.0f;
+            const float val = r < R ? (float)d : 0.0f;
+            const long tb = (tile0 + (long)(r >> 4) * NKB + (k >> 4)) * 256;
+            Tf[tb + 16 * (r & 15) + (k & 15)] = val;
+            Ti[tb + 16 * (k & 15) + (r & 15)] = val;
             if (r >= R - 1) continue;
             const double idx = M + r;
             const double nrm = sqrt((2.0 * idx + 3.0) / (2.0 * idx + 1.0));
@@ -93,86 +107,114 @@ __global__ void k_wigner_table_f32(PlanTables t, const long* __restrict__ roff, 
     }
 }
 
-constexpr int DPAD = 4;  // LDS row padding of the staged table chunk (floats)
+// Block -> (pair, signal group).  Blocks are dealt to the 8 XCDs round-robin,
+// so the signal groups of one pair are given consecutive slots of one XCD and
+// share its L2 copy of the pair's table tiles.
+struct BlockWork {
+    int pi, sg;
+};
 
-// Stage degrees 16 ch .. 16 ch + 15 of the pair's table into LDS.
-template <int NN, int NT>
-__device__ __forceinline__ void stage_table(float* Dc, const float* __restrict__ Dpair, int ch)
+__device__ __forceinline__ BlockWork block_work(int nsg)
 {
-    const float* src = Dpair + (long)ch * 16 * NN;
-    for (int e = threadIdx.x; e < 16 * NN; e += NT) {
-        const int r = e / NN, k = e - r * NN;
-        Dc[r * (NN + DPAD) + k] = src[e];
-    }
+    const int b = blockIdx.x, x = b & 7, tb = b >> 3;
+    BlockWork bw;
+    bw.sg = tb % nsg;
+    bw.pi = 8 * (tb / nsg) + x;
+    return bw;
 }
 
+// Forward contraction of one canonical pair for SPW signals per wave:
+// C[r][j] = sum_k D[r][k] * w_k * G_s[pencil(j/2)][k or N-1-k].(re|im by j&1),
+// as v_mfma_f32_16x16x4f32 over 16-degree chunks (A = table tile rows, read
+// from the row-major tiles) and 16-beta blocks (B = the signal's 8 case
+// vectors, read straight from the pencils).  Within a 16-beta block, MFMA
+// step t contracts betas 16 kb + 4 kk + t over the lane groups kk.
 template <int NN, int WPB, int SPW, int LAYOUT>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
-                                                                const float* __restrict__ D,
+                                                                const float* __restrict__ wts32,
+                                                                const float* __restrict__ Tf,
                                                                 const float2* __restrict__ G, float2* __restrict__ coeffs,
-                                                                int nsig, long count, float norm)
+                                                                int nsig, int nsg, long count, float norm)
 {
-    constexpr int NT = 64 * WPB;
-    extern __shared__ __align__(16) float smf[];
-    float* Dc = smf;                                                  // 16 x (NN + DPAD)
-    float2* St = reinterpret_cast<float2*>(smf + 16 * (NN + DPAD));   // [WPB][SPW][8][NN]
-    const int B = t.B;
+    constexpr int NCH = (NN / 2 + 15) / 16;
+    constexpr int NKB = NN / 16;
+    const BlockWork bw = block_work(nsg);
+    if (bw.pi >= t.npairs) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s0 = (bw.sg * WPB + w) * SPW;
+    if (s0 >= nsig) return;
+    const int2 pr = t.pairs[bw.pi];
+    const int M = pr.x, Mp = pr.y, R = t.B - M, nch = (R + 15) >> 4;
+    const int i = lane & 15, kk = lane >> 4, part = i & 1;
+    const BCase cb = bcase(i >> 1, M, Mp, NN);
     const long N3 = (long)NN * NN * NN;
-    const int2 pr = t.pairs[blockIdx.y];
-    const int M = pr.x, Mp = pr.y, R = B - M, nch = (R + 15) >> 4;
-    const float* Dpair = D + roff[blockIdx.y] * NN;
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int s0 = (blockIdx.x * WPB + w) * SPW;
-    float2* Sw = St + (long)w * SPW * 8 * NN;
 
-    // stage this wave's signals: weighted, reversed for the d(pi - beta) cases
-    for (int sp = 0; sp < SPW; ++sp) {
-        const int s = s0 + sp;
-        for (int c = 0; c < 8; ++c) {
-            const BCase ci = bcase(c, M, Mp, NN);
-            for (int k = lane; k < NN; k += 64) {
-                float2 v = make_float2(0.0f, 0.0f);
-                if (s < nsig && ci.active) {
-                    v = G[s * N3 + ci.vec + (ci.flip ? NN - 1 - k : k)];
-                    const float wk = (float)t.wts[k];
-                    v.x *= wk;
-                    v.y *= wk;
+    const float* gp[SPW];
+#pragma unroll
+    for (int sp = 0; sp < SPW; ++sp)
+        gp[sp] = reinterpret_cast<const float*>(G + (long)min(s0 + sp, nsig - 1) * N3 + cb.vec);
+    const float* tf = Tf + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
+
+    v4f acc[NCH][SPW];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) acc[ch][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    // lane's operand for step t of beta block kb: element k0 + 16 kb * dir + t * dir
+    // of its case pencil (dir = -1 for the reversed-beta cases), re or im part
+    const int dir = cb.flip ? -1 : 1;
+    const int k0 = cb.flip ? NN - 1 - 4 * kk : 4 * kk;
+    for (int kb = 0; kb < NKB; ++kb) {
+        const float4 wv = *reinterpret_cast<const float4*>(wts32 + 16 * kb + 4 * kk);
+        const int e0 = 2 * (k0 + 16 * kb * dir) + part, de = 2 * dir;
+        float4 bv[SPW];
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) {
+            bv[sp].x = gp[sp][e0];
+            bv[sp].y = gp[sp][e0 + de];
+            bv[sp].z = gp[sp][e0 + 2 * de];
+            bv[sp].w = gp[sp][e0 + 3 * de];
+        }
+        float4 a[NCH];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+            if (ch < nch) a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) {
+            bv[sp].x *= wv.x;
+            bv[sp].y *= wv.y;
+            bv[sp].z *= wv.z;
+            bv[sp].w *= wv.w;
+        }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            if (ch < nch) {
+#pragma unroll
+                for (int sp = 0; sp < SPW; ++sp) {
+                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].x, bv[sp].x, acc[ch][sp], 0, 0, 0);
+                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].y, bv[sp].y, acc[ch][sp], 0, 0, 0);
+                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].z, bv[sp].z, acc[ch][sp], 0, 0, 0);
+                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].w, bv[sp].w, acc[ch][sp], 0, 0, 0);
                 }
-                Sw[(sp * 8 + c) * NN + k] = v;
             }
         }
     }
-    // per-lane output case (column j = 2 case + re/im)
-    const int j = lane & 15, kk = lane >> 4;
-    const BCase co = bcase(j >> 1, M, Mp, NN);
 
-    for (int ch = 0; ch < nch; ++ch) {
-        __syncthreads();
-        stage_table<NN, NT>(Dc, Dpair, ch);
-        __syncthreads();
-        for (int sp = 0; sp < SPW; ++sp) {
-            const int s = s0 + sp;
-            if (s >= nsig) break;  // wave-uniform
-            const float2* Ss = Sw + (sp * 8 + (j >> 1)) * NN;
-            v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 8
-            for (int m = 0; m < NN / 4; ++m) {
-                const float a = Dc[(lane & 15) * (NN + DPAD) + 4 * m + kk];
-                const float2 bv = Ss[4 * m + kk];
-                const float b = (j & 1) ? bv.y : bv.x;
-                if (m & 1) acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc1, 0, 0, 0);
-                else acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc0, 0, 0, 0);
-            }
-            const v4f acc = acc0 + acc1;
-            if (co.active) {
-                float* cs = reinterpret_cast<float*>(coeffs + s * count) + (j & 1);
+    if (!cb.active) return;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = ch * 16 + out_row<LAYOUT>(lane, v);
-                    if (r < R) {
-                        const float sg = (float)((co.flip && (r & 1)) ? -co.sgn : co.sgn);
-                        cs[2 * coeff_index32(M + r, co.Mc, co.Mpc)] = norm * (sg * acc[v]);
-                    }
+    for (int sp = 0; sp < SPW; ++sp) {
+        if (s0 + sp >= nsig) break;
+        float* cs = reinterpret_cast<float*>(coeffs + (long)(s0 + sp) * count) + part;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            if (ch >= nch) break;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ch * 16 + out_row<LAYOUT>(lane, v);
+                if (r < R) {
+                    const float sg = (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
+                    cs[2 * coeff_index32(M + r, cb.Mc, cb.Mpc)] = norm * (sg * acc[ch][sp][v]);
                 }
             }
         }
@@ -181,65 +223,68 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
 
 constexpr int OSTRF = 17;  // output staging row stride (floats), NN rows of 16
 
+// Inverse contraction of one canonical pair for SPW signals per wave:
+// P[k][j] = sum_r D[r][k] * c_s(M + r, case j/2).(re|im by j&1) * sign * norm,
+// A = column-major table tiles (k rows, r columns), B = the coefficients,
+// gathered per lane.  The 8 beta-vectors of each signal are staged per wave
+// in LDS and written out as contiguous pencils.
 template <int NN, int WPB, int SPW, int LAYOUT>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, const long* __restrict__ roff,
-                                                                const float* __restrict__ D,
+                                                                const float* __restrict__ Ti,
                                                                 const float2* __restrict__ coeffs, float2* __restrict__ G,
-                                                                int nsig, long count, float norm)
+                                                                int nsig, int nsg, long count, float norm)
 {
-    constexpr int NT = 64 * WPB;
     constexpr int NKB = NN / 16;
     extern __shared__ __align__(16) float smf[];
-    float* Dc = smf;                                      // 16 x (NN + DPAD)
-    float* stage = smf + 16 * (NN + DPAD);                // [WPB][NN][OSTRF]
-    const int B = t.B;
+    const BlockWork bw = block_work(nsg);
+    if (bw.pi >= t.npairs) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s0 = (bw.sg * WPB + w) * SPW;
+    if (s0 >= nsig) return;
+    const int2 pr = t.pairs[bw.pi];
+    const int M = pr.x, Mp = pr.y, R = t.B - M, nch = (R + 15) >> 4;
+    const int i = lane & 15, kk = lane >> 4, part = i & 1;
+    const BCase cb = bcase(i >> 1, M, Mp, NN);
     const long N3 = (long)NN * NN * NN;
-    const int2 pr = t.pairs[blockIdx.y];
-    const int M = pr.x, Mp = pr.y, R = B - M, nch = (R + 15) >> 4;
-    const float* Dpair = D + roff[blockIdx.y] * NN;
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int s0 = (blockIdx.x * WPB + w) * SPW;
-    const int j = lane & 15, bsel = lane >> 4;
-    const BCase ci = bcase(j >> 1, M, Mp, NN);
 
-    v4f acc[SPW][NKB];
+    const float* cs[SPW];
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp)
-#pragma unroll
-        for (int kb = 0; kb < NKB; ++kb) acc[sp][kb] = (v4f){0.f, 0.f, 0.f, 0.f};
+        cs[sp] = reinterpret_cast<const float*>(coeffs + (long)min(s0 + sp, nsig - 1) * count) + part;
+    const float* ti = Ti + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
 
-    for (int ch = 0; ch < nch; ++ch) {
-        __syncthreads();
-        stage_table<NN, NT>(Dc, Dpair, ch);
-        __syncthreads();
+    v4f acc[NKB][SPW];
 #pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) {
-            const int s = s0 + sp;
-            if (s >= nsig) break;  // wave-uniform
-            float bv[4];
-            const float* cs = reinterpret_cast<const float*>(coeffs + s * count) + (j & 1);
+    for (int kb = 0; kb < NKB; ++kb)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = ch * 16 + 4 * q + bsel;
-                float v = 0.f;
-                if (ci.active && r < R) {
-                    const float sg = (float)((ci.flip && (r & 1)) ? -ci.sgn : ci.sgn);
-                    v = sg * (norm * cs[2 * coeff_index32(M + r, ci.Mc, ci.Mpc)]);
-                }
-                bv[q] = v;
+        for (int sp = 0; sp < SPW; ++sp) acc[kb][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    for (int q = 0; q < nch; ++q) {
+        float bv[SPW][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = 16 * q + 4 * kk + u;
+            const bool live = cb.active && r < R;
+            const int ci = live ? 2 * coeff_index32(M + r, cb.Mc, cb.Mpc) : 0;
+            const float sg = norm * (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
+#pragma unroll
+            for (int sp = 0; sp < SPW; ++sp) bv[sp][u] = live ? sg * cs[sp][ci] : 0.0f;
+        }
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            const float4 a = *reinterpret_cast<const float4*>(ti + (q * NKB + kb) * 256);
+#pragma unroll
+            for (int sp = 0; sp < SPW; ++sp) {
+                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bv[sp][0], acc[kb][sp], 0, 0, 0);
+                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bv[sp][1], acc[kb][sp], 0, 0, 0);
+                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, bv[sp][2], acc[kb][sp], 0, 0, 0);
+                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, bv[sp][3], acc[kb][sp], 0, 0, 0);
             }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int kb = 0; kb < NKB; ++kb) {
-                    const float a = Dc[(4 * q + bsel) * (NN + DPAD) + 16 * kb + j];
-                    acc[sp][kb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[q], acc[sp][kb], 0, 0, 0);
-                }
         }
     }
 
     // write each signal's 8 beta-vectors, staged so every case run is contiguous
-    float* sw = stage + (long)w * NN * OSTRF;
+    float* sw = smf + (long)w * NN * OSTRF;
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp) {
         const int s = s0 + sp;
@@ -247,14 +292,15 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sw[(16 * kb + out_row<LAYOUT>(lane, v)) * OSTRF + j] = acc[sp][kb][v];
+            for (int v = 0; v < 4; ++v) sw[(16 * kb + out_row<LAYOUT>(lane, v)) * OSTRF + i] = acc[kb][sp][v];
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
         for (int c = 0; c < 8; ++c) {
             const BCase co = bcase(c, M, Mp, NN);
             if (!co.active) continue;
+            float2* dst = G + (long)s * N3 + co.vec;
             for (int k = lane; k < NN; k += 64)
-                G[s * N3 + co.vec + (co.flip ? NN - 1 - k : k)] = make_float2(sw[k * OSTRF + 2 * c], sw[k * OSTRF + 2 * c + 1]);
+                dst[co.flip ? NN - 1 - k : k] = make_float2(sw[k * OSTRF + 2 * c], sw[k * OSTRF + 2 * c + 1]);
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
@@ -276,77 +322,91 @@ __global__ void k_nyquist_zero_f32(float2* __restrict__ G, int B, int nsig)
     G[s * (long)N * N * N + ((long)p * N + q) * N + k] = make_float2(0.f, 0.f);
 }
 
-template <int NN> struct BatchShape;
-template <> struct BatchShape<16> { static constexpr int WPB = 4, SPW = 4; };
-template <> struct BatchShape<32> { static constexpr int WPB = 4, SPW = 4; };
-template <> struct BatchShape<64> { static constexpr int WPB = 4, SPW = 2; };
-template <> struct BatchShape<128> { static constexpr int WPB = 4, SPW = 2; };
-template <> struct BatchShape<256> { static constexpr int WPB = 2, SPW = 1; };
+template <int NN> struct BatchShape;  // waves per block, signals per wave (forward, inverse)
+template <> struct BatchShape<16> { static constexpr int WPB = 4, SPWF = 4, SPWI = 4; };
+template <> struct BatchShape<32> { static constexpr int WPB = 4, SPWF = 4, SPWI = 4; };
+template <> struct BatchShape<64> { static constexpr int WPB = 4, SPWF = 4, SPWI = 4; };
+template <> struct BatchShape<128> { static constexpr int WPB = 4, SPWF = 4, SPWI = 2; };
+template <> struct BatchShape<256> { static constexpr int WPB = 4, SPWF = 2, SPWI = 2; };
+
+inline dim3 batch_grid(int npairs, int nsg) { return dim3((unsigned)(8 * ((npairs + 7) / 8) * nsg)); }
 
 template <int NN>
-hipError_t fwd_nn(const PlanTables& t, const long* roff, const float* D, const float2* G, float2* coeffs, int nsig,
-                  long count, float norm, hipStream_t st)
+hipError_t fwd_nn(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long count,
+                  float norm, hipStream_t st)
 {
-    constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPW;
-    const size_t smem = sizeof(float) * 16 * (NN + DPAD) + sizeof(float2) * (size_t)WPB * SPW * 8 * NN;
-    if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    dim3 grid((nsig + WPB * SPW - 1) / (WPB * SPW), t.npairs), block(64 * WPB);
+    constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPWF;
+    const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
+    const dim3 grid = batch_grid(t.npairs, nsg), block(64 * WPB);
     if (g_mfma_f32_layout == 0)
-        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 0>), grid, block, smem, st, t, roff, D, G, coeffs, nsig, count, norm);
+        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 0>), grid, block, 0, st, t, bt.roff, bt.wts32, bt.tf, G,
+                           coeffs, nsig, nsg, count, norm);
     else
-        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 1>), grid, block, smem, st, t, roff, D, G, coeffs, nsig, count, norm);
+        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 1>), grid, block, 0, st, t, bt.roff, bt.wts32, bt.tf, G,
+                           coeffs, nsig, nsg, count, norm);
     return hipGetLastError();
 }
 
 template <int NN>
-hipError_t inv_nn(const PlanTables& t, const long* roff, const float* D, const float2* coeffs, float2* G, int nsig,
-                  long count, float norm, hipStream_t st)
+hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long count,
+                  float norm, hipStream_t st)
 {
-    constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPW;
-    const size_t smem = sizeof(float) * (16 * (NN + DPAD) + (size_t)WPB * NN * OSTRF);
+    constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPWI;
+    const size_t smem = sizeof(float) * (size_t)WPB * NN * OSTRF;
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    dim3 grid((nsig + WPB * SPW - 1) / (WPB * SPW), t.npairs), block(64 * WPB);
+    const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
+    const dim3 grid = batch_grid(t.npairs, nsg), block(64 * WPB);
     if (g_mfma_f32_layout == 0)
-        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 0>), grid, block, smem, st, t, roff, D, coeffs, G, nsig, count, norm);
+        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 0>), grid, block, smem, st, t, bt.roff, bt.ti, coeffs, G,
+                           nsig, nsg, count, norm);
     else
-        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 1>), grid, block, smem, st, t, roff, D, coeffs, G, nsig, count, norm);
+        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 1>), grid, block, smem, st, t, bt.roff, bt.ti, coeffs, G,
+                           nsig, nsg, count, norm);
     return hipGetLastError();
 }
 
 }  // namespace
 
+long batch_table_rows(int B)
+{
+    long rows = 0;
+    for (int M = 0; M < B; ++M) rows += (long)(M + 1) * (((B - M + 15) / 16) * 16);
+    return rows;
+}
+
 bool batch_supported(int N) { return N == 16 || N == 32 || N == 64 || N == 128 || N == 256; }
 
-hipError_t launch_wigner_table_f32(const PlanTables& t, const long* roff, float* D, hipStream_t st)
+hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_wigner_table_f32, dim3(t.npairs), dim3(t.N < 256 ? t.N : 256), 0, st, t, roff, D);
+    hipLaunchKernelGGL(k_wigner_tiles_f32, dim3(t.npairs), dim3(t.N < 256 ? t.N : 256), 0, st, t, bt.roff,
+                       (float*)bt.wts32, (float*)bt.tf, (float*)bt.ti);
     return hipGetLastError();
 }
 
-hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const long* roff, const float* D, const float2* G,
-                                    float2* coeffs, int nsig, float norm, hipStream_t st)
+hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
+                                    int nsig, float norm, hipStream_t st)
 {
     const long count = coeff_offset(t.B);
     switch (t.N) {
-    case 16: return fwd_nn<16>(t, roff, D, G, coeffs, nsig, count, norm, st);
-    case 32: return fwd_nn<32>(t, roff, D, G, coeffs, nsig, count, norm, st);
-    case 64: return fwd_nn<64>(t, roff, D, G, coeffs, nsig, count, norm, st);
-    case 128: return fwd_nn<128>(t, roff, D, G, coeffs, nsig, count, norm, st);
-    case 256: return fwd_nn<256>(t, roff, D, G, coeffs, nsig, count, norm, st);
+    case 16: return fwd_nn<16>(t, bt, G, coeffs, nsig, count, norm, st);
+    case 32: return fwd_nn<32>(t, bt, G, coeffs, nsig, count, norm, st);
+    case 64: return fwd_nn<64>(t, bt, G, coeffs, nsig, count, norm, st);
+    case 128: return fwd_nn<128>(t, bt, G, coeffs, nsig, count, norm, st);
+    case 256: return fwd_nn<256>(t, bt, G, coeffs, nsig, count, norm, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const long* roff, const float* D, const float2* coeffs,
-                                    float2* G, int nsig, float norm, hipStream_t st)
+hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
+                                    int nsig, float norm, hipStream_t st)
 {
     const long count = coeff_offset(t.B);
     switch (t.N) {
-    case 16: return inv_nn<16>(t, roff, D, coeffs, G, nsig, count, norm, st);
-    case 32: return inv_nn<32>(t, roff, D, coeffs, G, nsig, count, norm, st);
-    case 64: return inv_nn<64>(t, roff, D, coeffs, G, nsig, count, norm, st);
-    case 128: return inv_nn<128>(t, roff, D, coeffs, G, nsig, count, norm, st);
-    case 256: return inv_nn<256>(t, roff, D, coeffs, G, nsig, count, norm, st);
+    case 16: return inv_nn<16>(t, bt, coeffs, G, nsig, count, norm, st);
+    case 32: return inv_nn<32>(t, bt, coeffs, G, nsig, count, norm, st);
+    case 64: return inv_nn<64>(t, bt, coeffs, G, nsig, count, norm, st);
+    case 128: return inv_nn<128>(t, bt, coeffs, G, nsig, count, norm, st);
+    case 256: return inv_nn<256>(t, bt, coeffs, G, nsig, count, norm, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -222,10 +222,9 @@ struct Plan {
     void* mem = nullptr;  // one allocation for all tables
     size_t bytes = 0;
     // FP32 Wigner tables of the batched path (built on first use)
-    void* d32 = nullptr;  // row offsets (long, per pair) followed by the table
+    void* d32 = nullptr;  // row offsets, FP32 weights and both tile tables
     size_t d32_bytes = 0;
-    const long* roff = nullptr;
-    const float* tab32 = nullptr;
+    BatchTables bt{};
 };
 
 }  // namespace
@@ -595,15 +594,23 @@ int ensure_table32(pfs_context* c, Plan& p, hipStream_t st)
         roff[i] = rows;
         rows += ((B - pairs[i].x + 15) / 16) * 16;
     }
-    const size_t head = ((pairs.size() * sizeof(long)) + 255) & ~(size_t)255;
-    const size_t bytes = head + (size_t)rows * N * sizeof(float);
+    if (rows != batch_table_rows(B)) return fail(PFS_ERR_ARG, "batched table: row count mismatch");
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_w = al(pairs.size() * sizeof(long));
+    const size_t o_tf = o_w + al(N * sizeof(float));
+    const size_t tab = (size_t)rows * N * sizeof(float);
+    const size_t o_ti = o_tf + al(tab);
+    const size_t bytes = o_ti + al(tab);
     int rc = ctx_alloc(c, &p.d32, bytes);
     if (rc) return rc;
     p.d32_bytes = bytes;
-    p.roff = (const long*)p.d32;
-    p.tab32 = (const float*)((char*)p.d32 + head);
-    HIP_TRY(hipMemcpyAsync((void*)p.roff, roff.data(), pairs.size() * sizeof(long), hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_wigner_table_f32(p.t, p.roff, (float*)p.tab32, st));
+    char* base = (char*)p.d32;
+    p.bt.roff = (const long*)base;
+    p.bt.wts32 = (const float*)(base + o_w);
+    p.bt.tf = (const float*)(base + o_tf);
+    p.bt.ti = (const float*)(base + o_ti);
+    HIP_TRY(hipMemcpyAsync((void*)p.bt.roff, roff.data(), pairs.size() * sizeof(long), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_wigner_tiles_f32(p.t, p.bt, st));
     HIP_TRY(hipStreamSynchronize(st));
     return PFS_OK;
 }
@@ -657,7 +664,7 @@ int dsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* grids, float2
         const int n = std::min(sb, nsig - s0);
         HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(grids + s0 * N3, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
         HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, G, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
-        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->roff, p->tab32, G, coeffs + s0 * count, n, norm, st); }));
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs + s0 * count, n, norm, st); }));
     }
     return PFS_OK;
 }
@@ -679,7 +686,7 @@ int idsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* coeffs, floa
     for (int s0 = 0; s0 < nsig; s0 += sb) {
         const int n = std::min(sb, nsig - s0);
         HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(G, B, n, st); }));
-        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->roff, p->tab32, coeffs + s0 * count, G, n, norm, st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs + s0 * count, G, n, norm, st); }));
         HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
         HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids + s0 * N3, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
     }

@@ -95,11 +95,20 @@ hipError_t launch_dwt_inv_f64_large(const PlanTables& t, const double2* coeffs, 
 
 // ---- batched mixed-precision path (batch.hip) ------------------------------
 bool batch_supported(int N);
-hipError_t launch_wigner_table_f32(const PlanTables& t, const long* roff, float* D, hipStream_t st);
-hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const long* roff, const float* D, const float2* G,
-                                    float2* coeffs, int nsig, float norm, hipStream_t st);
-hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const long* roff, const float* D, const float2* coeffs,
-                                    float2* G, int nsig, float norm, hipStream_t st);
+// FP32 tables of the batched path for one bandwidth (layout in batch.hip).
+struct BatchTables {
+    const long* roff;    // per pair: first table row (multiple of 16)
+    const float* wts32;  // quadrature weights rounded to FP32, N values
+    const float* tf;     // Wigner-d tiles, row-major within a tile
+    const float* ti;     // the same tiles, column-major within a tile
+};
+// Number of table rows (16-padded per pair) for bandwidth B.
+long batch_table_rows(int B);
+hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st);
+hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
+                                    int nsig, float norm, hipStream_t st);
+hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
+                                    int nsig, float norm, hipStream_t st);
 hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st);
 
 // Copy the grid entries the inverse DWT never writes (row == B or col == B)

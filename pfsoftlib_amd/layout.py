@@ -63,3 +63,60 @@ def seeded_coeffs(B, seed, power=1.0):
         out[coeff_offset(l) + iM + m * iMp] = blk
         pos += cnt
     return out
+
+
+def _fill_order(B, power):
+    """Packed destination index and scale of every fill-order position."""
+    n = coeff_count(B)
+    dest = np.empty(n, dtype=np.int64)
+    scale = np.empty(n, dtype=np.float64)
+    pos = 0
+    for l in range(B):
+        m = 2 * l + 1
+        cnt = m * m
+        Mo = np.repeat(np.arange(-l, l + 1), m)
+        Mpo = np.tile(np.arange(-l, l + 1), m)
+        iM = np.where(Mo >= 0, Mo, m + Mo)
+        iMp = np.where(Mpo >= 0, Mpo, m + Mpo)
+        dest[pos:pos + cnt] = coeff_offset(l) + iM + m * iMp
+        scale[pos:pos + cnt] = float((l + 1) ** (-power)) if power != 1.0 else 1.0 / (l + 1)
+        pos += cnt
+    return dest, scale
+
+
+def _i64(u):
+    """uint64 constant as the int64 with the same bits."""
+    u = int(u)
+    return u - (1 << 64) if u >= (1 << 63) else u
+
+
+def seeded_coeffs_torch(B, seeds, device, power=1.0, dtype=None):
+    """The generator of seeded_coeffs, vectorised over signals in torch.
+
+    Returns a (len(seeds), coeff_count(B)) complex128 tensor (or `dtype`) whose
+    row i equals seeded_coeffs(B, seeds[i], power) bit for bit.  SplitMix64 runs
+    in int64 with wrapping multiplication; logical shifts are arithmetic shifts
+    masked to the low bits."""
+    import torch
+    n = coeff_count(B)
+    dest, scale = _fill_order(B, power)
+    dest_t = torch.from_numpy(dest).to(device)
+    scale_t = torch.from_numpy(scale).to(device)
+    t = torch.arange(1, 2 * n + 1, dtype=torch.int64, device=device)
+    golden, m1, m2 = _i64(_GOLDEN), _i64(_M1), _i64(_M2)
+
+    def lsr(z, k):
+        return (z >> k) & ((1 << (64 - k)) - 1)
+
+    out = torch.empty((len(seeds), n), dtype=dtype or torch.complex128, device=device)
+    for i, seed in enumerate(seeds):
+        z = t * golden + _i64(seed % (1 << 64))
+        z = (z ^ lsr(z, 30)) * m1
+        z = (z ^ lsr(z, 27)) * m2
+        z = z ^ lsr(z, 31)
+        u = lsr(z, 11).to(torch.float64) * 2.0 ** -53
+        v = (2.0 * u - 1.0).view(n, 2)
+        row = torch.empty(n, dtype=torch.complex128, device=device)
+        row[dest_t] = torch.complex(v[:, 0] * scale_t, v[:, 1] * scale_t)
+        out[i] = row
+    return out

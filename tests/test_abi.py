@@ -69,6 +69,17 @@ def test_generator_matches_oracle_generator():
         assert np.array_equal(P.seeded_coeffs(B, 314, pw), O.seeded_coeffs(B, 314, pw))
 
 
+def test_torch_generator_is_bit_identical():
+    import torch
+    import oracle_bind as O
+    from pfsoftlib_amd.layout import seeded_coeffs_torch
+    seeds = [0, 11, 2 ** 63 + 5]
+    for B, pw in [(1, 1.0), (7, 1.0), (20, 1.5)]:
+        got = seeded_coeffs_torch(B, seeds, "cpu", pw).numpy()
+        for i, s in enumerate(seeds):
+            assert np.array_equal(got[i].view(np.uint64), O.seeded_coeffs(B, s, pw).view(np.uint64))
+
+
 def test_api_mirror_shape_checks_warn_and_return(capsys):
     import pfsoftlib_amd as P
     fc = P.DSOFTFourierCoefficients(4)
