@@ -125,18 +125,133 @@ __device__ __forceinline__ BlockWork block_work(int nsg)
 
 // Forward contraction of one canonical pair for SPW signals per wave:
 // C[r][j] = sum_k D[r][k] * w_k * G_s[pencil(j/2)][k or N-1-k].(re|im by j&1),
-// as v_mfma_f32_16x16x4f32 over 16-degree chunks (A = table tile rows, read
+// as v_mfma_f32_16x16x4f32 over NC 16-degree chunks (A = table tile rows, read
 // from the row-major tiles) and 16-beta blocks (B = the signal's 8 case
 // vectors, read straight from the pencils).  Within a 16-beta block, MFMA
-// step t contracts betas 16 kb + 4 kk + t over the lane groups kk.
+// step t contracts betas 16 kb + 4 kk + t over the lane groups kk.  The loads
+// of block kb + 1 are issued before the MFMAs of block kb (two register
+// stages), so memory latency overlaps the contraction.
+template <int NN, int SPW, int NC>
+struct FwdStage {
+    float4 a[NC];
+    float b[SPW][4];
+    float4 w;
+};
+
+template <int NN, int SPW, int NC>
+__device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC>& S, const float* const (&gp)[SPW], const float* tf,
+                                         const float* wl, int e0, int de, int kb)
+{
+    constexpr int NKB = NN / 16;
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) S.a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
+    const int e = e0 + 16 * kb * de;
+#pragma unroll
+    for (int sp = 0; sp < SPW; ++sp) {
+        S.b[sp][0] = gp[sp][e];
+        S.b[sp][1] = gp[sp][e + de];
+        S.b[sp][2] = gp[sp][e + 2 * de];
+        S.b[sp][3] = gp[sp][e + 3 * de];
+    }
+    S.w = *reinterpret_cast<const float4*>(wl + 16 * kb);
+}
+
+template <int NN, int SPW, int NC>
+__device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN, SPW, NC>& S)
+{
+    float b[SPW][4];
+#pragma unroll
+    for (int sp = 0; sp < SPW; ++sp) {
+        b[sp][0] = S.b[sp][0] * S.w.x;
+        b[sp][1] = S.b[sp][1] * S.w.y;
+        b[sp][2] = S.b[sp][2] * S.w.z;
+        b[sp][3] = S.b[sp][3] * S.w.w;
+    }
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch)
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) {
+            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].x, b[sp][0], acc[ch][sp], 0, 0, 0);
+            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].y, b[sp][1], acc[ch][sp], 0, 0, 0);
+            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].z, b[sp][2], acc[ch][sp], 0, 0, 0);
+            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].w, b[sp][3], acc[ch][sp], 0, 0, 0);
+        }
+}
+
+template <int NN, int SPW, int NC, int LAYOUT>
+__device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const float* tf, const float* wl, int e0,
+                                         int de, float2* __restrict__ coeffs, const BCase& cb, int M, int R, int s0,
+                                         int nsig, long cdist, long cstride, float norm, int lane)
+{
+    constexpr int NKB = NN / 16;
+    v4f acc[NC][SPW];
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch)
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) acc[ch][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    FwdStage<NN, SPW, NC> S0, S1;
+    fwd_load(S0, gp, tf, wl, e0, de, 0);
+    if constexpr (NKB == 1) {
+        fwd_mfma(acc, S0);
+    } else {
+        // The scheduling barriers keep each stage's loads ahead of the other
+        // stage's MFMAs (left alone, the scheduler sinks them next to their use).
+        for (int kb = 0; kb < NKB; kb += 2) {
+            fwd_load(S1, gp, tf, wl, e0, de, kb + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            fwd_mfma(acc, S0);
+            __builtin_amdgcn_sched_barrier(0);
+            // unconditional (the last one re-reads block kb): a branch here would
+            // make the compiler drain every outstanding load before the next MFMAs
+            fwd_load(S0, gp, tf, wl, e0, de, kb + 2 < NKB ? kb + 2 : kb);
+            __builtin_amdgcn_sched_barrier(0);
+            fwd_mfma(acc, S1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    if (!cb.active) return;
+    const int part = lane & 1;
+#pragma unroll
+    for (int sp = 0; sp < SPW; ++sp) {
+        if (s0 + sp >= nsig) break;
+        float* cs = reinterpret_cast<float*>(coeffs + (long)(s0 + sp) * cdist) + part;
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ch * 16 + out_row<LAYOUT>(lane, v);
+                if (r < R) {
+                    const float sg = (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
+                    cs[2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc)] = norm * (sg * acc[ch][sp][v]);
+                }
+            }
+    }
+}
+
+// Run fwd_pair with the pair's chunk count as a template argument.
+template <int NN, int SPW, int LAYOUT, int NC>
+__device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[SPW], const float* tf, const float* wl,
+                                             int e0, int de, float2* __restrict__ coeffs, const BCase& cb, int M, int R,
+                                             int s0, int nsig, long cdist, long cstride, float norm, int lane)
+{
+    if constexpr (NC <= (NN / 2 + 15) / 16) {
+        if (nch == NC)
+            fwd_pair<NN, SPW, NC, LAYOUT>(gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane);
+        else
+            fwd_dispatch<NN, SPW, LAYOUT, NC + 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride,
+                                                  norm, lane);
+    }
+}
+
 template <int NN, int WPB, int SPW, int LAYOUT>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
                                                                 const float* __restrict__ wts32,
                                                                 const float* __restrict__ Tf,
                                                                 const float2* __restrict__ G, float2* __restrict__ coeffs,
-                                                                int nsig, int nsg, long count, float norm)
+                                                                int nsig, int nsg, long cdist, long cstride, float norm)
 {
-    constexpr int NCH = (NN / 2 + 15) / 16;
     constexpr int NKB = NN / 16;
     const BlockWork bw = block_work(nsg);
     if (bw.pi >= t.npairs) return;
@@ -145,7 +260,7 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
     if (s0 >= nsig) return;
     const int2 pr = t.pairs[bw.pi];
     const int M = pr.x, Mp = pr.y, R = t.B - M, nch = (R + 15) >> 4;
-    const int i = lane & 15, kk = lane >> 4, part = i & 1;
+    const int i = lane & 15, kk = lane >> 4;
     const BCase cb = bcase(i >> 1, M, Mp, NN);
     const long N3 = (long)NN * NN * NN;
 
@@ -154,74 +269,65 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
     for (int sp = 0; sp < SPW; ++sp)
         gp[sp] = reinterpret_cast<const float*>(G + (long)min(s0 + sp, nsig - 1) * N3 + cb.vec);
     const float* tf = Tf + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
-
-    v4f acc[NCH][SPW];
-#pragma unroll
-    for (int ch = 0; ch < NCH; ++ch)
-#pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) acc[ch][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
-
-    // lane's operand for step t of beta block kb: element k0 + 16 kb * dir + t * dir
+    const float* wl = wts32 + 4 * kk;
+    // lane's operand for step t of beta block kb: element k0 + dir (16 kb + t)
     // of its case pencil (dir = -1 for the reversed-beta cases), re or im part
     const int dir = cb.flip ? -1 : 1;
     const int k0 = cb.flip ? NN - 1 - 4 * kk : 4 * kk;
-    for (int kb = 0; kb < NKB; ++kb) {
-        const float4 wv = *reinterpret_cast<const float4*>(wts32 + 16 * kb + 4 * kk);
-        const int e0 = 2 * (k0 + 16 * kb * dir) + part, de = 2 * dir;
-        float4 bv[SPW];
-#pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) {
-            bv[sp].x = gp[sp][e0];
-            bv[sp].y = gp[sp][e0 + de];
-            bv[sp].z = gp[sp][e0 + 2 * de];
-            bv[sp].w = gp[sp][e0 + 3 * de];
-        }
-        float4 a[NCH];
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch)
-            if (ch < nch) a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
-#pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) {
-            bv[sp].x *= wv.x;
-            bv[sp].y *= wv.y;
-            bv[sp].z *= wv.z;
-            bv[sp].w *= wv.w;
-        }
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-            if (ch < nch) {
-#pragma unroll
-                for (int sp = 0; sp < SPW; ++sp) {
-                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].x, bv[sp].x, acc[ch][sp], 0, 0, 0);
-                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].y, bv[sp].y, acc[ch][sp], 0, 0, 0);
-                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].z, bv[sp].z, acc[ch][sp], 0, 0, 0);
-                    acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ch].w, bv[sp].w, acc[ch][sp], 0, 0, 0);
-                }
-            }
-        }
-    }
+    const int e0 = 2 * k0 + (i & 1), de = 2 * dir;
 
-    if (!cb.active) return;
-#pragma unroll
-    for (int sp = 0; sp < SPW; ++sp) {
-        if (s0 + sp >= nsig) break;
-        float* cs = reinterpret_cast<float*>(coeffs + (long)(s0 + sp) * count) + part;
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-            if (ch >= nch) break;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = ch * 16 + out_row<LAYOUT>(lane, v);
-                if (r < R) {
-                    const float sg = (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
-                    cs[2 * coeff_index32(M + r, cb.Mc, cb.Mpc)] = norm * (sg * acc[ch][sp][v]);
-                }
-            }
-        }
-    }
+    fwd_dispatch<NN, SPW, LAYOUT, 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane);
 }
 
 constexpr int OSTRF = 17;  // output staging row stride (floats), NN rows of 16
+
+// Coefficient operands of one degree chunk as loaded (v) and the per-row
+// factor norm * sign (0 past R and for inactive cases) applied at use, so the
+// loads can stay in flight across the previous chunk's MFMAs.
+template <int SPW>
+struct InvStage {
+    float v[SPW][4];
+    float f[4];
+};
+
+template <int SPW>
+__device__ __forceinline__ void inv_gather(InvStage<SPW>& S, const float* const (&cs)[SPW], const BCase& cb, int M,
+                                           int R, long cstride, float norm, int kk, int q)
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = 16 * q + 4 * kk + u;
+        const bool live = cb.active && r < R;
+        const long ci = live ? 2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc) : 0;
+        S.f[u] = live ? norm * (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn) : 0.0f;
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) S.v[sp][u] = cs[sp][ci];
+    }
+}
+
+template <int NKB, int SPW>
+__device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStage<SPW>& S, const float* tq)
+{
+    float4 a[NKB];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) a[kb] = *reinterpret_cast<const float4*>(tq + kb * 256);
+    float b[SPW][4];
+#pragma unroll
+    for (int sp = 0; sp < SPW; ++sp)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[sp][u] = S.v[sp][u] * S.f[u];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) {
+            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].x, b[sp][0], acc[kb][sp], 0, 0, 0);
+            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].y, b[sp][1], acc[kb][sp], 0, 0, 0);
+            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].z, b[sp][2], acc[kb][sp], 0, 0, 0);
+            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].w, b[sp][3], acc[kb][sp], 0, 0, 0);
+        }
+    }
+}
 
 // Inverse contraction of one canonical pair for SPW signals per wave:
 // P[k][j] = sum_r D[r][k] * c_s(M + r, case j/2).(re|im by j&1) * sign * norm,
@@ -232,7 +338,7 @@ template <int NN, int WPB, int SPW, int LAYOUT>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, const long* __restrict__ roff,
                                                                 const float* __restrict__ Ti,
                                                                 const float2* __restrict__ coeffs, float2* __restrict__ G,
-                                                                int nsig, int nsg, long count, float norm)
+                                                                int nsig, int nsg, long cdist, long cstride, float norm)
 {
     constexpr int NKB = NN / 16;
     extern __shared__ __align__(16) float smf[];
@@ -250,7 +356,7 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
     const float* cs[SPW];
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp)
-        cs[sp] = reinterpret_cast<const float*>(coeffs + (long)min(s0 + sp, nsig - 1) * count) + part;
+        cs[sp] = reinterpret_cast<const float*>(coeffs + (long)min(s0 + sp, nsig - 1) * cdist) + part;
     const float* ti = Ti + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
 
     v4f acc[NKB][SPW];
@@ -259,28 +365,18 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
 #pragma unroll
         for (int sp = 0; sp < SPW; ++sp) acc[kb][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
 
+    // Coefficient operands of degree chunk q are gathered one chunk ahead of
+    // the chunk's MFMAs.  The loads are unconditional (coefficient 0 stands in
+    // past R and for inactive cases, scaled by 0): a branch around them would
+    // make the compiler drain all outstanding loads.
+    InvStage<SPW> bc, bn;
+    inv_gather<SPW>(bc, cs, cb, M, R, cstride, norm, kk, 0);
     for (int q = 0; q < nch; ++q) {
-        float bv[SPW][4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int r = 16 * q + 4 * kk + u;
-            const bool live = cb.active && r < R;
-            const int ci = live ? 2 * coeff_index32(M + r, cb.Mc, cb.Mpc) : 0;
-            const float sg = norm * (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
-#pragma unroll
-            for (int sp = 0; sp < SPW; ++sp) bv[sp][u] = live ? sg * cs[sp][ci] : 0.0f;
-        }
-#pragma unroll
-        for (int kb = 0; kb < NKB; ++kb) {
-            const float4 a = *reinterpret_cast<const float4*>(ti + (q * NKB + kb) * 256);
-#pragma unroll
-            for (int sp = 0; sp < SPW; ++sp) {
-                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bv[sp][0], acc[kb][sp], 0, 0, 0);
-                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bv[sp][1], acc[kb][sp], 0, 0, 0);
-                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, bv[sp][2], acc[kb][sp], 0, 0, 0);
-                acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, bv[sp][3], acc[kb][sp], 0, 0, 0);
-            }
-        }
+        inv_gather<SPW>(bn, cs, cb, M, R, cstride, norm, kk, q + 1 < nch ? q + 1 : q);
+        __builtin_amdgcn_sched_barrier(0);
+        inv_contract<NKB, SPW>(acc, bc, ti + q * NKB * 256);
+        __builtin_amdgcn_sched_barrier(0);
+        bc = bn;
     }
 
     // write each signal's 8 beta-vectors, staged so every case run is contiguous
@@ -304,6 +400,44 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Coefficient sets between the per-signal layout [s][count] (s < n) and the
+// signal-interleaved layout [idx][ld] the contraction kernels use, so that
+// their per-coefficient accesses cover the signals of a workgroup in one
+// contiguous run.  64 coefficients x 32 signals per workgroup, through LDS.
+template <bool TO_INTERLEAVED>
+__global__ __launch_bounds__(256) void k_coeff_transpose(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                         long count, int n, int ld)
+{
+    __shared__ float2 tile[32][65];
+    const long i0 = (long)blockIdx.x * 64;
+    const int s0 = blockIdx.y * 32, t = threadIdx.x;
+    if (TO_INTERLEAVED) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int sl = (t >> 6) + 4 * r, il = t & 63;
+            if (s0 + sl < n && i0 + il < count) tile[sl][il] = src[(long)(s0 + sl) * count + i0 + il];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int il = (t >> 5) + 8 * r, sl = t & 31;
+            if (s0 + sl < n && i0 + il < count) dst[(i0 + il) * ld + s0 + sl] = tile[sl][il];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int il = (t >> 5) + 8 * r, sl = t & 31;
+            if (s0 + sl < n && i0 + il < count) tile[sl][il] = src[(i0 + il) * ld + s0 + sl];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int sl = (t >> 6) + 4 * r, il = t & 63;
+            if (s0 + sl < n && i0 + il < count) dst[(long)(s0 + sl) * count + i0 + il] = tile[sl][il];
+        }
     }
 }
 
@@ -332,24 +466,24 @@ template <> struct BatchShape<256> { static constexpr int WPB = 4, SPWF = 2, SPW
 inline dim3 batch_grid(int npairs, int nsg) { return dim3((unsigned)(8 * ((npairs + 7) / 8) * nsg)); }
 
 template <int NN>
-hipError_t fwd_nn(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long count,
-                  float norm, hipStream_t st)
+hipError_t fwd_nn(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long cdist,
+                  long cstride, float norm, hipStream_t st)
 {
     constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPWF;
     const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
     const dim3 grid = batch_grid(t.npairs, nsg), block(64 * WPB);
     if (g_mfma_f32_layout == 0)
         hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 0>), grid, block, 0, st, t, bt.roff, bt.wts32, bt.tf, G,
-                           coeffs, nsig, nsg, count, norm);
+                           coeffs, nsig, nsg, cdist, cstride, norm);
     else
         hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 1>), grid, block, 0, st, t, bt.roff, bt.wts32, bt.tf, G,
-                           coeffs, nsig, nsg, count, norm);
+                           coeffs, nsig, nsg, cdist, cstride, norm);
     return hipGetLastError();
 }
 
 template <int NN>
-hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long count,
-                  float norm, hipStream_t st)
+hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
+                  long cstride, float norm, hipStream_t st)
 {
     constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPWI;
     const size_t smem = sizeof(float) * (size_t)WPB * NN * OSTRF;
@@ -358,10 +492,10 @@ hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coef
     const dim3 grid = batch_grid(t.npairs, nsg), block(64 * WPB);
     if (g_mfma_f32_layout == 0)
         hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 0>), grid, block, smem, st, t, bt.roff, bt.ti, coeffs, G,
-                           nsig, nsg, count, norm);
+                           nsig, nsg, cdist, cstride, norm);
     else
         hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 1>), grid, block, smem, st, t, bt.roff, bt.ti, coeffs, G,
-                           nsig, nsg, count, norm);
+                           nsig, nsg, cdist, cstride, norm);
     return hipGetLastError();
 }
 
@@ -384,31 +518,41 @@ hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, h
 }
 
 hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
-                                    int nsig, float norm, hipStream_t st)
+                                    int nsig, long cdist, long cstride, float norm, hipStream_t st)
 {
-    const long count = coeff_offset(t.B);
     switch (t.N) {
-    case 16: return fwd_nn<16>(t, bt, G, coeffs, nsig, count, norm, st);
-    case 32: return fwd_nn<32>(t, bt, G, coeffs, nsig, count, norm, st);
-    case 64: return fwd_nn<64>(t, bt, G, coeffs, nsig, count, norm, st);
-    case 128: return fwd_nn<128>(t, bt, G, coeffs, nsig, count, norm, st);
-    case 256: return fwd_nn<256>(t, bt, G, coeffs, nsig, count, norm, st);
+    case 16: return fwd_nn<16>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    case 32: return fwd_nn<32>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    case 64: return fwd_nn<64>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    case 128: return fwd_nn<128>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    case 256: return fwd_nn<256>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
-                                    int nsig, float norm, hipStream_t st)
+                                    int nsig, long cdist, long cstride, float norm, hipStream_t st)
 {
-    const long count = coeff_offset(t.B);
     switch (t.N) {
-    case 16: return inv_nn<16>(t, bt, coeffs, G, nsig, count, norm, st);
-    case 32: return inv_nn<32>(t, bt, coeffs, G, nsig, count, norm, st);
-    case 64: return inv_nn<64>(t, bt, coeffs, G, nsig, count, norm, st);
-    case 128: return inv_nn<128>(t, bt, coeffs, G, nsig, count, norm, st);
-    case 256: return inv_nn<256>(t, bt, coeffs, G, nsig, count, norm, st);
+    case 16: return inv_nn<16>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    case 32: return inv_nn<32>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    case 64: return inv_nn<64>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    case 128: return inv_nn<128>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    case 256: return inv_nn<256>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_coeff_transpose(const float2* src, float2* dst, long count, int n, int ld, bool to_interleaved,
+                                  hipStream_t st)
+{
+    if (n < 1 || ld < n) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((count + 63) / 64), (unsigned)((n + 31) / 32));
+    if (to_interleaved)
+        hipLaunchKernelGGL(k_coeff_transpose<true>, grid, dim3(256), 0, st, src, dst, count, n, ld);
+    else
+        hipLaunchKernelGGL(k_coeff_transpose<false>, grid, dim3(256), 0, st, src, dst, count, n, ld);
+    return hipGetLastError();
 }
 
 hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st)

@@ -255,6 +255,7 @@ struct pfs_context {
     // tuning knobs (pfs_tune_set); 0 = automatic
     long fft_chunk_slices = 0;
     long batch_chunk_signals = 64;
+    long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -647,6 +648,14 @@ int batch_prepare(pfs_context* c, int B, int nsig, Plan** pp, int* sb, hipStream
     return ensure_bwork(c, (size_t)*sb * N * N * N * sizeof(float2));
 }
 
+// Row length of the interleaved coefficient layout for a sub-batch of n
+// signals: padded to 8 signals (32-byte runs) when the work buffer has room.
+int interleave_ld(int n, int sb, long count, long N3)
+{
+    const int ld = (n + 7) & ~7;
+    return (long)ld * count <= (long)sb * N3 ? ld : n;
+}
+
 int dsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* grids, float2* coeffs, hipStream_t st)
 {
     Plan* p;
@@ -664,7 +673,14 @@ int dsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* grids, float2
         const int n = std::min(sb, nsig - s0);
         HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(grids + s0 * N3, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
         HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, G, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
-        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs + s0 * count, n, norm, st); }));
+        if (c->batch_interleave) {
+            // W is free once the column pass has run: the coefficients go there interleaved
+            const int ld = interleave_ld(n, sb, count, N3);
+            HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, st); }));
+            HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(W, coeffs + s0 * count, count, n, ld, false, st); }));
+        } else {
+            HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs + s0 * count, n, count, 1, norm, st); }));
+        }
     }
     return PFS_OK;
 }
@@ -686,7 +702,13 @@ int idsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* coeffs, floa
     for (int s0 = 0; s0 < nsig; s0 += sb) {
         const int n = std::min(sb, nsig - s0);
         HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(G, B, n, st); }));
-        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs + s0 * count, G, n, norm, st); }));
+        if (c->batch_interleave) {
+            const int ld = interleave_ld(n, sb, count, N3);
+            HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs + s0 * count, W, count, n, ld, true, st); }));
+            HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, st); }));
+        } else {
+            HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs + s0 * count, G, n, count, 1, norm, st); }));
+        }
         HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
         HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids + s0 * N3, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
     }
@@ -998,6 +1020,11 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->batch_chunk_signals = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "batch_interleave")) {
+        if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "batch_interleave must be 0 or 1");
+        c->batch_interleave = value;
+        return PFS_OK;
+    }
     return fail(PFS_ERR_ARG, "pfs_tune_set: unknown key '%s'", key);
 }
 
@@ -1011,6 +1038,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_chunk_signals")) {
         *value = c->batch_chunk_signals;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_interleave")) {
+        *value = c->batch_interleave;
         return PFS_OK;
     }
     return fail(PFS_ERR_ARG, "pfs_tune_get: unknown key '%s'", key);

@@ -318,8 +318,12 @@ template <int Rn, int SIGN, typename C> __device__ __forceinline__ void dft_smal
     else dft2<SIGN>(v);
 }
 
+// Lines per workgroup of the register kernel: 128 bytes of one column element
+// per line group, so the tile-side accesses are whole 128-byte runs.
+template <typename T> constexpr int r16_lines() { return 128 / (2 * (int)sizeof(T)); }
+
 template <typename T, int LOGN, int KIND, int SIGN>
-__global__ __launch_bounds__(8 * (1 << LOGN) / 16) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
+__global__ __launch_bounds__(r16_lines<T>() * (1 << LOGN) / 16) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
                                                                           typename CT<T>::C* __restrict__ out,
                                                                           LineMap m,
                                                                           const typename CT<T>::C* __restrict__ tw,
@@ -328,7 +332,7 @@ __global__ __launch_bounds__(8 * (1 << LOGN) / 16) void k_fft_lines_r16(const ty
     typedef typename CT<T>::C C;
     constexpr int N = 1 << LOGN;
     constexpr int TPL = N / 16;          // threads per line; also the step-B DFT size
-    constexpr int L = 8;                 // lines per workgroup
+    constexpr int L = r16_lines<T>();    // lines per workgroup
     constexpr int YS = TPL + 1;          // LDS row stride (complex) for one k2
     constexpr int LS = 16 * YS + 1;      // LDS stride between lines
     __shared__ C lds[L * LS];
@@ -390,10 +394,11 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
 {
     constexpr int N = 1 << LOGN;
     if (LOGN >= 6 && LOGN <= 8) {
-        if (m.nv % 8 != 0) return hipErrorInvalidValue;
-        dim3 grid(m.nv / 8, m.nu, nsig);
+        constexpr int L = r16_lines<T>();
+        if (m.nv % L != 0) return hipErrorInvalidValue;
+        dim3 grid(m.nv / L, m.nu, nsig);
         hipLaunchKernelGGL((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
-                           dim3(8 * N / 16), 0, st, in, out, m, tw, scale);
+                           dim3(L * N / 16), 0, st, in, out, m, tw, scale);
         return hipGetLastError();
     }
     constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;

@@ -105,10 +105,14 @@ struct BatchTables {
 // Number of table rows (16-padded per pair) for bandwidth B.
 long batch_table_rows(int B);
 hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st);
+// Coefficient (s, idx) of the batch sits at s * cdist + idx * cstride (complex
+// elements): (count, 1) is the per-signal layout, (1, ld) the interleaved one.
 hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
-                                    int nsig, float norm, hipStream_t st);
+                                    int nsig, long cdist, long cstride, float norm, hipStream_t st);
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
-                                    int nsig, float norm, hipStream_t st);
+                                    int nsig, long cdist, long cstride, float norm, hipStream_t st);
+hipError_t launch_coeff_transpose(const float2* src, float2* dst, long count, int n, int ld, bool to_interleaved,
+                                  hipStream_t st);
 hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st);
 
 // Copy the grid entries the inverse DWT never writes (row == B or col == B)

@@ -77,3 +77,29 @@ def test_batch_rejects_unsupported_bandwidth(ctx, torch):
     x = torch.zeros(8, dtype=torch.complex64, device="cuda")
     with pytest.raises(P.PfsError):
         ctx.dsoft_batch_f32(12, 1, x, x)
+
+
+@pytest.mark.parametrize("B,nsig,chunk", [(16, 3, 1), (32, 11, 5), (64, 3, 64)])
+def test_batch_interleaved_and_direct_coefficients_agree(ctx, torch, B, nsig, chunk):
+    # the contraction on signal-interleaved coefficients (with transposes) and on
+    # the caller's per-signal layout sum in the same order: results are identical
+    N = 2 * B
+    s = torch.cuda.current_stream()
+    cs = np.concatenate([O.seeded_coeffs(B, 40 + i) for i in range(nsig)]).astype(np.complex64)
+    cdev = torch.from_numpy(cs).cuda()
+    out = {}
+    ctx.tune("batch_chunk_signals", chunk)
+    for mode in (0, 1):
+        ctx.tune("batch_interleave", mode)
+        g = torch.empty(nsig * N ** 3, dtype=torch.complex64, device="cuda")
+        ctx.idsoft_batch_f32(B, nsig, cdev, g, stream=s)
+        c = torch.empty_like(cdev)
+        ctx.dsoft_batch_f32(B, nsig, g, c, stream=s)
+        out[mode] = (g, c)
+    torch.cuda.synchronize()
+    ctx.tune("batch_interleave", 1)
+    ctx.tune("batch_chunk_signals", 64)
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+    gref = O.idsoft(B, cs[:O.coeff_count(B)].astype(np.complex128))
+    assert rel(out[1][0][:N ** 3].cpu().numpy().astype(np.complex128), gref) < TOL
