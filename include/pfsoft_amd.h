@@ -168,6 +168,58 @@ int pfs_memcpy_d2h(pfs_context* ctx, void* dst_host, const void* src_dev, size_t
 /* Waits for stream; NULL waits for the context's own stream and the null stream. */
 int pfs_synchronize(pfs_context* ctx, void* stream);
 
+/* ---- request server ----------------------------------------------------
+ * No counterpart in the reference, whose transforms are blocking calls
+ * (include/PFSOFTlib_headers/fn_fourier_transforms.hpp:75,78) that are not
+ * safe to call concurrently (src/fftw_wrapper.cpp:58-59,83).  A server on a
+ * context accepts forward (complex64 grid -> packed complex64 coefficients)
+ * and inverse (coefficients -> grid, Nyquist entries 0) requests of mixed
+ * bandwidth on caller-owned device buffers, which must stay valid until the
+ * request completes, and runs them with continuous batching on the batched
+ * mixed-precision path: up to `slots` batches in flight on separate streams;
+ * a free slot takes the oldest waiting request plus the waiting requests of
+ * the same direction and bandwidth, up to its capacity.
+ * hbm_budget_bytes (0 = only max_batch limits) bounds the server's own device
+ * memory: resident FP32 tables of the served bandwidths plus the slots' work
+ * buffers (two complex64 grids per signal of a batch).  A result does not
+ * depend on which requests share its batch.  The server's streams are ordered
+ * after work enqueued on the HIP null stream before a submission; work on
+ * other streams that writes a request's input must be complete when it is
+ * submitted.  Submit and collect/wait may be called from any thread; a ticket
+ * is consumed by pfs_server_wait or by pfs_server_collect, not both. */
+typedef struct pfs_server pfs_server;
+
+enum pfs_request_kind {
+    PFS_REQ_FORWARD = 0, /* DSOFT:  in = grid, out = coefficients  */
+    PFS_REQ_INVERSE = 1  /* iDSOFT: in = coefficients, out = grid  */
+};
+
+typedef struct pfs_server_stats_t {
+    uint64_t completed;            /* requests finished (ok or failed)      */
+    uint64_t batches;              /* batches launched                      */
+    uint64_t signals;              /* signals over all batches              */
+    uint64_t pending;              /* requests waiting for a slot           */
+    int max_inflight_batches;      /* most batches on the device at once    */
+    size_t device_bytes;           /* tables + work buffers held            */
+    size_t slot_bytes;             /* work-buffer bytes per slot            */
+} pfs_server_stats_t;
+
+int pfs_server_create(pfs_context* ctx, const int* bandwidths, int nbw, int slots, int max_batch,
+                      size_t hbm_budget_bytes, pfs_server** out);
+/* Finishes every submitted request, then frees the server. */
+int pfs_server_destroy(pfs_server* srv);
+int pfs_server_submit(pfs_server* srv, int kind, int bandwidth, const void* in_dev, void* out_dev, uint64_t* ticket);
+/* Blocks until the request is done (timeout_s < 0: no limit); latency_s =
+ * completion - submission, host clock.  Returns the request's status. */
+int pfs_server_wait(pfs_server* srv, uint64_t ticket, double timeout_s, double* latency_s);
+/* Up to cap requests finished since the last call, oldest first. */
+int pfs_server_collect(pfs_server* srv, uint64_t* tickets, double* latency_s, int* status, int cap, int* n_out);
+/* Blocks until nothing is waiting or running. */
+int pfs_server_drain(pfs_server* srv, double timeout_s);
+int pfs_server_stats(pfs_server* srv, pfs_server_stats_t* out);
+/* Signals per batch the server admits at this bandwidth. */
+int pfs_server_capacity(pfs_server* srv, int bandwidth, int* signals_per_batch);
+
 #ifdef __cplusplus
 }
 #endif

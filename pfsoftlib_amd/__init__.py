@@ -29,6 +29,16 @@ PFS_F64 = 0
 PFS_F32 = 1
 
 
+PFS_REQ_FORWARD = 0
+PFS_REQ_INVERSE = 1
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("signals", ctypes.c_uint64),
+                ("pending", ctypes.c_uint64), ("max_inflight_batches", ctypes.c_int),
+                ("device_bytes", ctypes.c_size_t), ("slot_bytes", ctypes.c_size_t)]
+
+
 class PfsError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("pfsoft_amd error %d: %s" % (code, msg))
@@ -75,6 +85,15 @@ def lib():
             "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_memcpy_d2h": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_synchronize": (ip, [vp, vp]),
+            "pfs_server_create": (ip, [vp, ctypes.POINTER(ip), ip, ip, ip, ctypes.c_size_t, ctypes.POINTER(vp)]),
+            "pfs_server_destroy": (ip, [vp]),
+            "pfs_server_submit": (ip, [vp, ip, ip, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+            "pfs_server_wait": (ip, [vp, ctypes.c_uint64, ctypes.c_double, dp]),
+            "pfs_server_collect": (ip, [vp, ctypes.POINTER(ctypes.c_uint64), dp, ctypes.POINTER(ip), ip,
+                                        ctypes.POINTER(ip)]),
+            "pfs_server_drain": (ip, [vp, ctypes.c_double]),
+            "pfs_server_stats": (ip, [vp, ctypes.POINTER(ServerStats)]),
+            "pfs_server_capacity": (ip, [vp, ip, ctypes.POINTER(ip)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -327,5 +346,72 @@ def IDSOFT(fc, synthesis, threads=None, ctx=None):
     synthesis.mem[:] = (ctx or default_context()).idsoft(B, fc.mem, synthesis.mem)
 
 
+class Server:
+    """Request server on a context (pfs_server_* in the C ABI): continuous
+    batching of forward / inverse complex64 requests of mixed bandwidth on
+    caller-owned device buffers (torch tensors or device addresses)."""
+
+    def __init__(self, ctx, bandwidths, slots=4, max_batch=64, hbm_budget=0):
+        self.ctx = ctx
+        arr = (ctypes.c_int * len(bandwidths))(*bandwidths)
+        h = ctypes.c_void_p()
+        _check(lib().pfs_server_create(ctx._h, arr, len(bandwidths), slots, max_batch, hbm_budget, ctypes.byref(h)))
+        self._h = h
+        self._keep = {}  # ticket -> buffers, kept alive until the request is consumed
+
+    def submit(self, kind, B, inp, out):
+        t = ctypes.c_uint64()
+        _check(lib().pfs_server_submit(self._h, kind, B, _devptr(inp), _devptr(out), ctypes.byref(t)))
+        self._keep[t.value] = (inp, out)
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """Blocks until the request is done; returns its latency in seconds."""
+        lat = ctypes.c_double()
+        try:
+            _check(lib().pfs_server_wait(self._h, ticket, timeout, ctypes.byref(lat)))
+        finally:
+            self._keep.pop(ticket, None)
+        return lat.value
+
+    def collect(self, cap=4096):
+        """[(ticket, latency_s, status)] of the requests finished since the last call."""
+        tk = (ctypes.c_uint64 * cap)()
+        lat = (ctypes.c_double * cap)()
+        st = (ctypes.c_int * cap)()
+        n = ctypes.c_int()
+        _check(lib().pfs_server_collect(self._h, tk, lat, st, cap, ctypes.byref(n)))
+        out = [(tk[i], lat[i], st[i]) for i in range(n.value)]
+        for t, _, _ in out:
+            self._keep.pop(t, None)
+        return out
+
+    def drain(self, timeout=-1.0):
+        _check(lib().pfs_server_drain(self._h, timeout))
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().pfs_server_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def capacity(self, B):
+        n = ctypes.c_int()
+        _check(lib().pfs_server_capacity(self._h, B, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._h:
+            _check(lib().pfs_server_destroy(self._h))
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 __all__ = ["Context", "PfsError", "DSOFT", "IDSOFT", "DSOFTFourierCoefficients", "grid3D",
-           "coeff_count", "coeff_index", "coeff_offset", "seeded_coeffs", "layer_wise_DFT2_grid3D", "lib"]
+           "coeff_count", "coeff_index", "coeff_offset", "seeded_coeffs", "layer_wise_DFT2_grid3D", "lib",
+           "Server", "PFS_REQ_FORWARD", "PFS_REQ_INVERSE"]

@@ -134,10 +134,22 @@ __device__ __forceinline__ BlockWork block_work(int nsg)
 template <int NN, int SPW, int NC>
 struct FwdStage {
     float4 a[NC];
-    float b[SPW][4];
+    float4 g[SPW];  // two consecutive complex values of this lane's case pencil
     float4 w;
 };
 
+// Exchange with the partner lane (lane ^ 1) through DPP quad_perm [1,0,3,2].
+__device__ __forceinline__ float swap_pair(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+// Lane pair (2c, 2c+1) of lane group kk covers the four betas 16 kb + 4 kk + t
+// (t = 0..3) of case c: for the direct cases lane 2c+p loads the complex pair
+// at 16 kb + 4 kk + 2p, for the reversed cases the pair at N - 4 - 16 kb - 4 kk
+// + 2(1-p) -- one 16-byte load per lane, 1 KB contiguous per wave (8 pencils
+// x 128 bytes).  The partners then swap halves so that the even lane holds the
+// real parts and the odd lane the imaginary parts of the four betas, in t order.
 template <int NN, int SPW, int NC>
 __device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC>& S, const float* const (&gp)[SPW], const float* tf,
                                          const float* wl, int e0, int de, int kb)
@@ -145,27 +157,31 @@ __device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC>& S, const float* 
     constexpr int NKB = NN / 16;
 #pragma unroll
     for (int ch = 0; ch < NC; ++ch) S.a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
-    const int e = e0 + 16 * kb * de;
+    const int e = e0 + kb * de;
 #pragma unroll
-    for (int sp = 0; sp < SPW; ++sp) {
-        S.b[sp][0] = gp[sp][e];
-        S.b[sp][1] = gp[sp][e + de];
-        S.b[sp][2] = gp[sp][e + 2 * de];
-        S.b[sp][3] = gp[sp][e + 3 * de];
-    }
+    for (int sp = 0; sp < SPW; ++sp) S.g[sp] = *reinterpret_cast<const float4*>(gp[sp] + e);
     S.w = *reinterpret_cast<const float4*>(wl + 16 * kb);
 }
 
 template <int NN, int SPW, int NC>
-__device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN, SPW, NC>& S)
+__device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN, SPW, NC>& S, bool odd, bool flip)
 {
     float b[SPW][4];
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp) {
-        b[sp][0] = S.b[sp][0] * S.w.x;
-        b[sp][1] = S.b[sp][1] * S.w.y;
-        b[sp][2] = S.b[sp][2] * S.w.z;
-        b[sp][3] = S.b[sp][3] * S.w.w;
+        const float4 o = S.g[sp];
+        float4 x;
+        x.x = swap_pair(o.x);
+        x.y = swap_pair(o.y);
+        x.z = swap_pair(o.z);
+        x.w = swap_pair(o.w);
+        // direct:   even (o.x, o.z, x.x, x.z), odd (x.y, x.w, o.y, o.w)
+        // reversed: even (o.z, o.x, x.z, x.x), odd (x.w, x.y, o.w, o.y)
+        const float d0 = odd ? x.y : o.x, d1 = odd ? x.w : o.z, d2 = odd ? o.y : x.x, d3 = odd ? o.w : x.z;
+        b[sp][0] = (flip ? d1 : d0) * S.w.x;
+        b[sp][1] = (flip ? d0 : d1) * S.w.y;
+        b[sp][2] = (flip ? d3 : d2) * S.w.z;
+        b[sp][3] = (flip ? d2 : d3) * S.w.w;
     }
 #pragma unroll
     for (int ch = 0; ch < NC; ++ch)
@@ -190,23 +206,24 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
 #pragma unroll
         for (int sp = 0; sp < SPW; ++sp) acc[ch][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
 
+    const bool odd = lane & 1, flip = cb.flip;
     FwdStage<NN, SPW, NC> S0, S1;
     fwd_load(S0, gp, tf, wl, e0, de, 0);
     if constexpr (NKB == 1) {
-        fwd_mfma(acc, S0);
+        fwd_mfma(acc, S0, odd, flip);
     } else {
         // The scheduling barriers keep each stage's loads ahead of the other
         // stage's MFMAs (left alone, the scheduler sinks them next to their use).
         for (int kb = 0; kb < NKB; kb += 2) {
             fwd_load(S1, gp, tf, wl, e0, de, kb + 1);
             __builtin_amdgcn_sched_barrier(0);
-            fwd_mfma(acc, S0);
+            fwd_mfma(acc, S0, odd, flip);
             __builtin_amdgcn_sched_barrier(0);
             // unconditional (the last one re-reads block kb): a branch here would
             // make the compiler drain every outstanding load before the next MFMAs
             fwd_load(S0, gp, tf, wl, e0, de, kb + 2 < NKB ? kb + 2 : kb);
             __builtin_amdgcn_sched_barrier(0);
-            fwd_mfma(acc, S1);
+            fwd_mfma(acc, S1, odd, flip);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -270,11 +287,10 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
         gp[sp] = reinterpret_cast<const float*>(G + (long)min(s0 + sp, nsig - 1) * N3 + cb.vec);
     const float* tf = Tf + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
     const float* wl = wts32 + 4 * kk;
-    // lane's operand for step t of beta block kb: element k0 + dir (16 kb + t)
-    // of its case pencil (dir = -1 for the reversed-beta cases), re or im part
-    const int dir = cb.flip ? -1 : 1;
-    const int k0 = cb.flip ? NN - 1 - 4 * kk : 4 * kk;
-    const int e0 = 2 * k0 + (i & 1), de = 2 * dir;
+    // float offset of this lane's complex pair in beta block 0, and its step per block
+    const int p = i & 1;
+    const int e0 = 2 * (cb.flip ? NN - 4 - 4 * kk + 2 * (1 - p) : 4 * kk + 2 * p);
+    const int de = cb.flip ? -32 : 32;
 
     fwd_dispatch<NN, SPW, LAYOUT, 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane);
 }
@@ -407,36 +423,39 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
 // signal-interleaved layout [idx][ld] the contraction kernels use, so that
 // their per-coefficient accesses cover the signals of a workgroup in one
 // contiguous run.  64 coefficients x 32 signals per workgroup, through LDS.
+// The per-signal side is either contiguous (ps + s * count) or, with sptr
+// set, signal s at sptr[s].
 template <bool TO_INTERLEAVED>
-__global__ __launch_bounds__(256) void k_coeff_transpose(const float2* __restrict__ src, float2* __restrict__ dst,
-                                                         long count, int n, int ld)
+__global__ __launch_bounds__(256) void k_coeff_transpose(float2* __restrict__ ps, float2* const* __restrict__ sptr,
+                                                         float2* __restrict__ il, long count, int n, int ld)
 {
     __shared__ float2 tile[32][65];
     const long i0 = (long)blockIdx.x * 64;
     const int s0 = blockIdx.y * 32, t = threadIdx.x;
+    auto row = [&](int s) { return sptr ? sptr[s] : ps + (long)s * count; };
     if (TO_INTERLEAVED) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int sl = (t >> 6) + 4 * r, il = t & 63;
-            if (s0 + sl < n && i0 + il < count) tile[sl][il] = src[(long)(s0 + sl) * count + i0 + il];
+            const int sl = (t >> 6) + 4 * r, il_ = t & 63;
+            if (s0 + sl < n && i0 + il_ < count) tile[sl][il_] = row(s0 + sl)[i0 + il_];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int il = (t >> 5) + 8 * r, sl = t & 31;
-            if (s0 + sl < n && i0 + il < count) dst[(i0 + il) * ld + s0 + sl] = tile[sl][il];
+            const int ix = (t >> 5) + 8 * r, sl = t & 31;
+            if (s0 + sl < n && i0 + ix < count) il[(i0 + ix) * ld + s0 + sl] = tile[sl][ix];
         }
     } else {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int il = (t >> 5) + 8 * r, sl = t & 31;
-            if (s0 + sl < n && i0 + il < count) tile[sl][il] = src[(i0 + il) * ld + s0 + sl];
+            const int ix = (t >> 5) + 8 * r, sl = t & 31;
+            if (s0 + sl < n && i0 + ix < count) tile[sl][ix] = il[(i0 + ix) * ld + s0 + sl];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int sl = (t >> 6) + 4 * r, il = t & 63;
-            if (s0 + sl < n && i0 + il < count) dst[(long)(s0 + sl) * count + i0 + il] = tile[sl][il];
+            const int sl = (t >> 6) + 4 * r, il_ = t & 63;
+            if (s0 + sl < n && i0 + il_ < count) row(s0 + sl)[i0 + il_] = tile[sl][il_];
         }
     }
 }
@@ -543,15 +562,16 @@ hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, 
     }
 }
 
-hipError_t launch_coeff_transpose(const float2* src, float2* dst, long count, int n, int ld, bool to_interleaved,
-                                  hipStream_t st)
+hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float2* interleaved, long count, int n,
+                                  int ld, bool to_interleaved, hipStream_t st)
 {
     if (n < 1 || ld < n) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((count + 63) / 64), (unsigned)((n + 31) / 32));
     if (to_interleaved)
-        hipLaunchKernelGGL(k_coeff_transpose<true>, grid, dim3(256), 0, st, src, dst, count, n, ld);
+        hipLaunchKernelGGL(k_coeff_transpose<true>, grid, dim3(256), 0, st, per_signal, sptr, interleaved, count, n, ld);
     else
-        hipLaunchKernelGGL(k_coeff_transpose<false>, grid, dim3(256), 0, st, src, dst, count, n, ld);
+        hipLaunchKernelGGL(k_coeff_transpose<false>, grid, dim3(256), 0, st, per_signal, sptr, interleaved, count, n,
+                           ld);
     return hipGetLastError();
 }
 

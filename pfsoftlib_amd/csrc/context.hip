@@ -649,11 +649,63 @@ int batch_prepare(pfs_context* c, int B, int nsig, Plan** pp, int* sb, hipStream
 }
 
 // Row length of the interleaved coefficient layout for a sub-batch of n
-// signals: padded to 8 signals (32-byte runs) when the work buffer has room.
-int interleave_ld(int n, int sb, long count, long N3)
+// signals in work buffers sized for cap signals: padded to 8 signals (32-byte
+// runs) when the buffer has room.
+int interleave_ld(int n, int cap, long count, long N3)
 {
     const int ld = (n + 7) & ~7;
-    return (long)ld * count <= (long)sb * N3 ? ld : n;
+    return (long)ld * count <= (long)cap * N3 ? ld : n;
+}
+
+// Forward batch of n <= cap signals: FFT rows (grids -> W), FFT columns
+// (W -> G), DWT (G -> coefficients).  W and G hold cap grids each.
+int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSet coeffs, float2* W, float2* G,
+                    hipStream_t st)
+{
+    const int B = p->B, N = p->t.N;
+    const long N3 = (long)N * N * N, count = coeff_offset(B);
+    LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
+    a.in_sig = grids.stride;
+    a.in_ptrs = (const void* const*)grids.ptrs;
+    a.out_sig = b.in_sig = b.out_sig = N3;
+    const float norm = (float)fwd_norm(B);
+    HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(grids.base, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
+    HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, G, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
+    if (c->batch_interleave || coeffs.ptrs) {
+        // W is free once the column pass has run: the coefficients go there interleaved
+        const int ld = interleave_ld(n, cap, count, N3);
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, st); }));
+        HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs.base, coeffs.ptrs, W, count, n, ld, false, st); }));
+    } else {
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs.base, n, coeffs.stride, 1, norm, st); }));
+    }
+    return PFS_OK;
+}
+
+// Inverse batch of n <= cap signals: DWT (coefficients -> G, Nyquist pencils
+// zeroed), FFT columns (G -> W), FFT rows (W -> grids) x 1/(4B^2).
+int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, SigSet grids, float2* W, float2* G,
+                     hipStream_t st)
+{
+    const int B = p->B, N = p->t.N;
+    const long N3 = (long)N * N * N, count = coeff_offset(B);
+    LineMap b = map_cols_inv(N), a = map_rows_inv(N);
+    a.out_sig = grids.stride;
+    a.out_ptrs = (void* const*)grids.ptrs;
+    a.in_sig = b.in_sig = b.out_sig = N3;
+    const float norm = (float)inv_norm(B);
+    const float scale = (float)(1. / (4. * B * B));
+    HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(G, B, n, st); }));
+    if (c->batch_interleave || coeffs.ptrs) {
+        const int ld = interleave_ld(n, cap, count, N3);
+        HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs.base, coeffs.ptrs, W, count, n, ld, true, st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, st); }));
+    } else {
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, st); }));
+    }
+    HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
+    HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids.base, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
+    return PFS_OK;
 }
 
 int dsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* grids, float2* coeffs, hipStream_t st)
@@ -662,25 +714,12 @@ int dsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* grids, float2
     int sb;
     int rc = batch_prepare(c, B, nsig, &p, &sb, st);
     if (rc) return rc;
-    const int N = p->t.N;
-    const long N3 = (long)N * N * N, count = coeff_offset(B);
-    float2* W = (float2*)c->bwork[0];
-    float2* G = (float2*)c->bwork[1];
-    LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
-    a.in_sig = a.out_sig = b.in_sig = b.out_sig = N3;
-    const float norm = (float)fwd_norm(B);
+    const long N3 = (long)p->t.N * p->t.N * p->t.N, count = coeff_offset(B);
+    const SigSet in{const_cast<float2*>(grids), N3, nullptr}, out{coeffs, count, nullptr};
     for (int s0 = 0; s0 < nsig; s0 += sb) {
-        const int n = std::min(sb, nsig - s0);
-        HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(grids + s0 * N3, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
-        HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, G, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
-        if (c->batch_interleave) {
-            // W is free once the column pass has run: the coefficients go there interleaved
-            const int ld = interleave_ld(n, sb, count, N3);
-            HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, st); }));
-            HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(W, coeffs + s0 * count, count, n, ld, false, st); }));
-        } else {
-            HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs + s0 * count, n, count, 1, norm, st); }));
-        }
+        rc = dsoft_batch_run(c, p, std::min(sb, nsig - s0), sb, in.at(s0), out.at(s0), (float2*)c->bwork[0],
+                             (float2*)c->bwork[1], st);
+        if (rc) return rc;
     }
     return PFS_OK;
 }
@@ -691,26 +730,12 @@ int idsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* coeffs, floa
     int sb;
     int rc = batch_prepare(c, B, nsig, &p, &sb, st);
     if (rc) return rc;
-    const int N = p->t.N;
-    const long N3 = (long)N * N * N, count = coeff_offset(B);
-    float2* W = (float2*)c->bwork[0];
-    float2* G = (float2*)c->bwork[1];
-    LineMap b = map_cols_inv(N), a = map_rows_inv(N);
-    a.in_sig = a.out_sig = b.in_sig = b.out_sig = N3;
-    const float norm = (float)inv_norm(B);
-    const float scale = (float)(1. / (4. * B * B));
+    const long N3 = (long)p->t.N * p->t.N * p->t.N, count = coeff_offset(B);
+    const SigSet in{const_cast<float2*>(coeffs), count, nullptr}, out{grids, N3, nullptr};
     for (int s0 = 0; s0 < nsig; s0 += sb) {
-        const int n = std::min(sb, nsig - s0);
-        HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(G, B, n, st); }));
-        if (c->batch_interleave) {
-            const int ld = interleave_ld(n, sb, count, N3);
-            HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs + s0 * count, W, count, n, ld, true, st); }));
-            HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, st); }));
-        } else {
-            HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs + s0 * count, G, n, count, 1, norm, st); }));
-        }
-        HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
-        HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids + s0 * N3, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
+        rc = idsoft_batch_run(c, p, std::min(sb, nsig - s0), sb, in.at(s0), out.at(s0), (float2*)c->bwork[0],
+                              (float2*)c->bwork[1], st);
+        if (rc) return rc;
     }
     return PFS_OK;
 }
@@ -1137,3 +1162,51 @@ int pfs_synchronize(pfs_context* c, void* stream)
 }
 
 }  // extern "C"
+
+// ---- batched pipeline for the request server -------------------------------
+
+namespace pfs {
+
+int batch_warm(pfs_context* c, int B)
+{
+    std::lock_guard<std::mutex> g(c->mu);
+    Plan* p;
+    int rc = get_plan(c, B, &p);
+    if (rc) return rc;
+    if (!batch_supported(p->t.N))
+        return fail(PFS_ERR_UNSUPPORTED, "batched mixed-precision path supports B = 8, 16, 32, 64, 128 (got %d)", B);
+    return ensure_table32(c, *p, c->stream);
+}
+
+size_t batch_work_bytes(int B, int cap)
+{
+    const size_t N = 2 * (size_t)B;
+    return (size_t)cap * N * N * N * sizeof(float2);
+}
+
+int batch_run(pfs_context* c, bool forward, int B, int n, int cap, SigSet in, SigSet out, float2* W, float2* G,
+              hipStream_t st)
+{
+    std::lock_guard<std::mutex> g(c->mu);
+    if (n < 1 || n > cap) return fail(PFS_ERR_ARG, "batch_run: n = %d outside [1, %d]", n, cap);
+    auto it = c->plans.find(B);
+    if (it == c->plans.end() || !it->second.d32) return fail(PFS_ERR_ARG, "batch_run: bandwidth %d not warmed", B);
+    Plan* p = &it->second;
+    return forward ? dsoft_batch_run(c, p, n, cap, in, out, W, G, st) : idsoft_batch_run(c, p, n, cap, in, out, W, G, st);
+}
+
+}  // namespace pfs
+
+namespace pfs {
+
+int set_error(int code, const char* msg)
+{
+    return fail(code, "%s", msg);
+}
+
+const char* error_text() { return g_last_error.c_str(); }
+
+int context_device(const pfs_context* c) { return c->device; }
+
+}  // namespace pfs
+

@@ -193,6 +193,18 @@ __device__ __forceinline__ void store_tile(const C* lds, int ls, C* __restrict__
     }
 }
 
+// First element of signal sig on either side of a line map.
+template <typename C>
+__device__ __forceinline__ const C* sig_in(const C* in, const LineMap& m, long sig)
+{
+    return m.in_ptrs ? reinterpret_cast<const C*>(m.in_ptrs[sig]) : in + sig * m.in_sig;
+}
+template <typename C>
+__device__ __forceinline__ C* sig_out(C* out, const LineMap& m, long sig)
+{
+    return m.out_ptrs ? reinterpret_cast<C*>(m.out_ptrs[sig]) : out + sig * m.out_sig;
+}
+
 template <typename T, int LOGN, int LINES, int KIND, int SIGN>
 __global__ __launch_bounds__(256) void k_fft_lines(const typename CT<T>::C* __restrict__ in,
                                                    typename CT<T>::C* __restrict__ out, LineMap m,
@@ -204,12 +216,12 @@ __global__ __launch_bounds__(256) void k_fft_lines(const typename CT<T>::C* __re
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int ls = N + 1;
     const long u = blockIdx.y, v0 = (long)blockIdx.x * LINES, sig = blockIdx.z;
-    const long ib = sig * m.in_sig + u * m.in_su + v0 * m.in_sv;
-    const long ob = sig * m.out_sig + u * m.out_su + v0 * m.out_sv;
-    load_tile<KIND>(lds, ls, in, ib, m.in_sv, m.in_sn, N, LINES);
+    const long ib = u * m.in_su + v0 * m.in_sv;
+    const long ob = u * m.out_su + v0 * m.out_sv;
+    load_tile<KIND>(lds, ls, sig_in(in, m, sig), ib, m.in_sv, m.in_sn, N, LINES);
     __syncthreads();
     fft_in_lds<LOGN, LINES, SIGN>(lds, ls, tw);
-    store_tile<KIND>(lds, ls, out, ob, m.out_sv, m.out_sn, N, LINES, scale);
+    store_tile<KIND>(lds, ls, sig_out(out, m, sig), ob, m.out_sv, m.out_sn, N, LINES, scale);
 }
 
 // Direct DFT for even N that is not a power of two.
@@ -225,9 +237,10 @@ __global__ __launch_bounds__(256) void k_dft_lines(const typename CT<T>::C* __re
     const int ls = N + 1;
     C* b = a + lines * ls;
     const long u = blockIdx.y, v0 = (long)blockIdx.x * lines, sig = blockIdx.z;
-    const long ib = sig * m.in_sig + u * m.in_su + v0 * m.in_sv;
-    const long ob = sig * m.out_sig + u * m.out_su + v0 * m.out_sv;
-    load_tile<KIND>(a, ls, in, ib, m.in_sv, m.in_sn, N, lines);
+    const long ib = u * m.in_su + v0 * m.in_sv;
+    const long ob = u * m.out_su + v0 * m.out_sv;
+    out = sig_out(out, m, sig);
+    load_tile<KIND>(a, ls, sig_in(in, m, sig), ib, m.in_sv, m.in_sn, N, lines);
     __syncthreads();
     for (int e = threadIdx.x; e < lines * N; e += 256) {
         int line = e / N, k = e - line * N;
@@ -339,8 +352,8 @@ __global__ __launch_bounds__(r16_lines<T>() * (1 << LOGN) / 16) void k_fft_lines
 
     const int tid = threadIdx.x;
     const long u = blockIdx.y, v0 = (long)blockIdx.x * L, sig = blockIdx.z;
-    const C* src = in + sig * m.in_sig + u * m.in_su + v0 * m.in_sv;
-    C* dst = out + sig * m.out_sig + u * m.out_su + v0 * m.out_sv;
+    const C* src = sig_in(in, m, sig) + u * m.in_su + v0 * m.in_sv;
+    C* dst = sig_out(out, m, sig) + u * m.out_su + v0 * m.out_sv;
 
     // step A: rows read with n1 fastest, tiles with the line fastest
     int la, n1;

@@ -60,6 +60,18 @@ extern int g_mfma_f32_layout;
 // Largest dynamic LDS allocation a workgroup may request on the current device.
 size_t lds_limit();
 
+// Where the signals of a batch live: base + s * stride (complex elements), or,
+// when ptrs is set, at the device addresses ptrs[s] (a device array).
+struct SigSet {
+    float2* base;
+    long stride;
+    float2* const* ptrs;
+    SigSet at(int s0) const
+    {
+        return ptrs ? SigSet{nullptr, stride, ptrs + s0} : SigSet{base + (long)s0 * stride, stride, nullptr};
+    }
+};
+
 // ---- FFT stage -----------------------------------------------------------
 // A batch of length-N line transforms over a 3D array.  Line (u, v) element n
 // sits at u*su + v*sv + n*sn (complex elements) on each side; u runs over
@@ -69,6 +81,10 @@ struct LineMap {
     long out_su, out_sv, out_sn;
     long in_sig, out_sig;  // per-signal strides
     int nu, nv;            // extents of u and v
+    // when set, signal s of the input (output) starts at in_ptrs[s] (out_ptrs[s])
+    // instead of in + s * in_sig; device arrays of device addresses
+    const void* const* in_ptrs;
+    void* const* out_ptrs;
 };
 
 enum LineKind { LINES_ROW_IN_TILE_OUT = 0, LINES_TILE_IN_ROW_OUT = 1 };
@@ -111,12 +127,36 @@ hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, 
                                     int nsig, long cdist, long cstride, float norm, hipStream_t st);
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
                                     int nsig, long cdist, long cstride, float norm, hipStream_t st);
-hipError_t launch_coeff_transpose(const float2* src, float2* dst, long count, int n, int ld, bool to_interleaved,
-                                  hipStream_t st);
+// Between the per-signal layout (per_signal + s * count, or sptr[s] when sptr
+// is set) and the interleaved layout [idx][ld], in the direction given.
+hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float2* interleaved, long count, int n,
+                                  int ld, bool to_interleaved, hipStream_t st);
 hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st);
 
 // Copy the grid entries the inverse DWT never writes (row == B or col == B)
 // into the pencil buffer, so the inverse FFT reads the caller's values there.
 hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st);
 
+// ---- batched pipeline entry points used by the request server ------------
 }  // namespace pfs
+
+struct pfs_context;
+
+namespace pfs {
+// Plan and FP32 tables for bandwidth B (built once; takes the context lock).
+int batch_warm(pfs_context* c, int B);
+// Bytes of each of the two work buffers for cap signals at bandwidth B.
+size_t batch_work_bytes(int B, int cap);
+// One batched forward (grids -> coefficients) or inverse transform of n <= cap
+// complex64 signals on stream st with caller-owned work buffers W and G of
+// batch_work_bytes(B, cap) bytes each.  Takes the context lock while enqueueing.
+int batch_run(pfs_context* c, bool forward, int B, int n, int cap, SigSet in, SigSet out, float2* W, float2* G,
+              hipStream_t st);
+// Set this thread's pfs_last_error() text; returns code.
+int set_error(int code, const char* msg);
+// This thread's pfs_last_error() text.
+const char* error_text();
+// Device ordinal of a context.
+int context_device(const pfs_context* c);
+}  // namespace pfs
+
