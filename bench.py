@@ -163,6 +163,130 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
     }
 
 
+SERVE_BANDWIDTHS = [16, 32, 64, 128]
+SERVE_PROBS = [0.4, 0.3, 0.2, 0.1]
+
+
+def serving_leg(torch, P, ctx, seconds, seed, budget_bytes, slots, max_batch, util=0.7, rate=None):
+    """Seeded Poisson request stream through the request server.
+
+    Bandwidths {16, 32, 64, 128} with probabilities {0.4, 0.3, 0.2, 0.1};
+    50/50 forward DSOFT on an iDSOFT-made complex64 grid and inverse iDSOFT on
+    seeded coefficients; inputs resident in HBM, outputs into rings of device
+    buffers.  The offered rate is util x the capacity measured by a saturating
+    burst of the same mix (unless given).  Reports p50/p99 latency (submit to
+    completion, host clock), completed transforms/s and the error of every
+    ring slot's last result against the plain FP64 path on the same input."""
+    import numpy as np
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(seed)
+    nin = 4
+    ring = {16: 256, 32: 256, 64: 96, 128: 32}
+    inputs, outs, last = {}, {}, {}
+    for B in SERVE_BANDWIDTHS:
+        N, n = 2 * B, P.coeff_count(B)
+        c = P.seeded_coeffs_torch(B, [seed + 100 * B + i for i in range(nin)], "cuda", dtype=torch.complex64)
+        g = torch.empty((nin, N ** 3), dtype=torch.complex64, device="cuda")
+        ctx.idsoft_batch_f32(B, nin, c, g, stream=stream)
+        inputs[(B, P.PFS_REQ_INVERSE)] = c
+        inputs[(B, P.PFS_REQ_FORWARD)] = g
+        outs[(B, P.PFS_REQ_INVERSE)] = torch.empty((ring[B], N ** 3), dtype=torch.complex64, device="cuda")
+        outs[(B, P.PFS_REQ_FORWARD)] = torch.empty((ring[B], n), dtype=torch.complex64, device="cuda")
+    torch.cuda.synchronize()
+    srv = P.Server(ctx, SERVE_BANDWIDTHS, slots=slots, max_batch=max_batch, hbm_budget=budget_bytes)
+    pos = {key: 0 for key in outs}
+    busy = {}  # (key, ring slot) -> ticket not yet collected
+    owner = {}  # ticket -> (key, ring slot)
+    lat = []
+
+    def collect():
+        for t, l, st in srv.collect():
+            if st != 0:
+                raise RuntimeError("serving request failed with status %d" % st)
+            lat.append(l)
+            busy.pop(owner.pop(t), None)
+
+    def submit(i):
+        B = int(rng.choice(SERVE_BANDWIDTHS, p=SERVE_PROBS))
+        kind = int(rng.integers(0, 2))
+        key = (B, kind)
+        r = pos[key]
+        pos[key] = (r + 1) % ring[B]
+        while (key, r) in busy:  # ring slot still in flight
+            collect()
+        j = i % nin
+        t = srv.submit(kind, B, inputs[key][j], outs[key][r])
+        busy[(key, r)] = t
+        owner[t] = (key, r)
+        last[(key, r)] = j
+
+    # capacity: a saturating burst of the same mix
+    burst = 4000
+    t0 = time.perf_counter()
+    for i in range(burst):
+        submit(i)
+    srv.drain()
+    cap_rate = burst / (time.perf_counter() - t0)
+    collect()
+    lat.clear()
+    lam = rate if rate else util * cap_rate
+
+    # the Poisson stream
+    gaps = rng.exponential(1.0 / lam, size=int(lam * seconds * 1.2) + 16)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < seconds]
+    st0 = srv.stats()
+    t0 = time.perf_counter()
+    next_collect = t0
+    for i, ta in enumerate(arrivals):
+        while True:
+            now = time.perf_counter()
+            if now - t0 >= ta:
+                break
+            if now >= next_collect:
+                collect()
+                next_collect = now + 0.002
+        submit(i)
+    srv.drain()
+    t1 = time.perf_counter()
+    collect()
+    st1 = srv.stats()
+    done = len(lat)
+    lat_ms = sorted(1e3 * x for x in lat)
+
+    # error of each ring slot's last result against the FP64 path on its input
+    errs = {}
+    for (key, r), j in last.items():
+        B, kind = key
+        if len(errs.get(key, [])) >= 4:
+            continue
+        N = 2 * B
+        out = outs[key][r].to(torch.complex128)
+        if kind == P.PFS_REQ_INVERSE:
+            ref = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
+            ctx.idsoft_dev(B, inputs[key][j].to(torch.complex128), ref, stream=stream)
+        else:
+            ref = torch.empty(P.coeff_count(B), dtype=torch.complex128, device="cuda")
+            ctx.dsoft_dev(B, inputs[key][j].to(torch.complex128), ref, stream=stream)
+        e = (torch.linalg.vector_norm(out - ref) / torch.linalg.vector_norm(ref)).item()
+        errs.setdefault(key, []).append(e)
+    srv.close()
+    name = {P.PFS_REQ_FORWARD: "dsoft", P.PFS_REQ_INVERSE: "idsoft"}
+    return {
+        "bandwidths": SERVE_BANDWIDTHS, "probabilities": SERVE_PROBS, "forward_fraction": 0.5,
+        "hbm_budget_bytes": budget_bytes, "slots": slots, "max_batch": max_batch,
+        "capacity_per_s": cap_rate, "offered_rate_per_s": lam, "target_utilisation": util if not rate else None,
+        "requests": int(len(arrivals)), "completed": done, "seconds": t1 - t0,
+        "transforms_per_s": done / (t1 - t0),
+        "latency_ms": {"p50": lat_ms[len(lat_ms) // 2], "p99": lat_ms[min(len(lat_ms) - 1, int(0.99 * len(lat_ms)))],
+                       "max": lat_ms[-1]},
+        "batches": st1["batches"] - st0["batches"],
+        "mean_batch": (st1["signals"] - st0["signals"]) / max(1, st1["batches"] - st0["batches"]),
+        "max_inflight_batches": st1["max_inflight_batches"], "server_device_bytes": st1["device_bytes"],
+        "rel_l2_vs_fp64_max": {"%s_B%d" % (name[k[1]], k[0]): max(v) for k, v in sorted(errs.items())},
+    }
+
+
 def copy_calibration(torch, steps, mib):
     """A plain elementwise copy of known size, for scaling PMC byte counters."""
     n = mib * (1 << 20) // 8
@@ -189,7 +313,13 @@ def main():
                     help="signals per batched mixed-precision round trip at the same bandwidth (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-check", type=int, default=0, help="signals checked against FP64 (0 = all)")
-    ap.add_argument("--c4", action="store_true", help="also run 512 signals at B=64 through the batched path")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 512-signal B=64 batched leg")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="run the Poisson request stream through the request server for this long (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="offered requests/s (0 = 70%% of measured capacity)")
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -287,8 +417,11 @@ def main():
         out["batched_mixed"] = bl
         out["batched_vs_plain_fp64"] = bl["pairs_per_s"] / out["value"]
         torch.cuda.empty_cache()
-    if args.c4:
+    if not args.no_c4:
         out["batched_mixed_512xB64"] = batched_leg(torch, P, ctx, 64, 512, args.batch_steps, 1, args.seed, args.batch_check)
+    if args.serve_seconds > 0:
+        out["serving"] = serving_leg(torch, P, ctx, args.serve_seconds, args.seed, int(args.serve_budget_gb * 2 ** 30),
+                                     args.serve_slots, args.serve_max_batch, rate=args.serve_rate or None)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
     print(json.dumps(out))
