@@ -42,33 +42,73 @@ def algorithmic_bytes_per_step(kernel, B, e=16):
     return 0
 
 
+def measured_traffic(B, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/pmc_traffic_f64_B<B>.json), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_f64_B%d.json" % B)
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(B, pairs, seed):
+    """The oracle (C/OpenMP restatement of the reference's DSOFT/IDSOFT) on this
+    host, following the reference's benchmark protocol
+    (benchmark/benchmark_dsoft_for.cpp:44-48,135-223): warm the OpenMP pool,
+    build the input once, time forward and inverse separately over `pairs`
+    round trips with all host threads (avg/min/max), then one round trip with
+    one thread for the serial time, speedup and parallel efficiency."""
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
     import numpy as np
     import oracle_bind as O
     threads = O.lib().oracle_max_threads()
     c = O.seeded_coeffs(B, seed)
     O.dsoft(4, O.idsoft(4, O.seeded_coeffs(4, 1)))  # warm the OpenMP pool
-    t_inv, t_fwd = [], []
-    g = None
-    for _ in range(pairs):
-        t0 = time.perf_counter()
-        g = O.idsoft(B, c, threads=threads)
-        t1 = time.perf_counter()
-        r = O.dsoft(B, g, threads=threads)
-        t2 = time.perf_counter()
-        t_inv.append(t1 - t0)
-        t_fwd.append(t2 - t1)
+
+    def run(n, th):
+        t_inv, t_fwd, r = [], [], None
+        for _ in range(n):
+            t0 = time.perf_counter()
+            g = O.idsoft(B, c, threads=th)
+            t1 = time.perf_counter()
+            r = O.dsoft(B, g, threads=th)
+            t2 = time.perf_counter()
+            t_inv.append(t1 - t0)
+            t_fwd.append(t2 - t1)
+        return t_inv, t_fwd, r
+
+    t_inv, t_fwd, r = run(pairs, threads)
     err = float(np.linalg.norm(r - c) / np.linalg.norm(c))
-    tot = sum(t_inv) + sum(t_fwd)
+    s_inv, s_fwd, _ = run(1, 1)
+    par = (sum(t_inv) + sum(t_fwd)) / pairs
+    ser = s_inv[0] + s_fwd[0]
     return {
-        "value": pairs / tot,
+        "value": 1.0 / par,
         "unit": "DSOFT+iDSOFT pairs/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%d full round trips at B=%d (same seeded input), oracle C/OpenMP restatement of the "
-                  "reference (long double DWT, own long double FFT in place of FFTW); fwd avg %.3f s, inv avg %.3f s"
-                  % (pairs, B, sum(t_fwd) / pairs, sum(t_inv) / pairs),
+        "sample": "%d round trips at B=%d with %d threads plus 1 with 1 thread (same seeded input); oracle "
+                  "C/OpenMP restatement of the reference (long double DWT, own long double FFT in place of FFTW)"
+                  % (pairs, B, threads),
+        "cpu_model": cpu_model(),
+        "forward_s": {"avg": sum(t_fwd) / pairs, "min": min(t_fwd), "max": max(t_fwd)},
+        "inverse_s": {"avg": sum(t_inv) / pairs, "min": min(t_inv), "max": max(t_inv)},
+        "serial_pair_s": ser,
+        "speedup": ser / par,
+        "efficiency": ser / par / threads,
         "roundtrip_rel_l2": err,
     }
 
@@ -306,7 +346,7 @@ def main():
     ap.add_argument("--bandwidth", type=int, default=128)
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-pairs", type=int, default=2)
+    ap.add_argument("--cpu-pairs", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--fft-chunk", type=int, default=0, help="beta-slices per FFT chunk (0 = automatic)")
     ap.add_argument("--batch-signals", type=int, default=64,
@@ -397,7 +437,8 @@ def main():
                                "%d coefficients, inputs resident in HBM" % (B, N, P.coeff_count(B)),
                    "bandwidth": B, "grid": [N, N, N], "precision": "fp64", "signals_per_step": 1},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": None},
+                     "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": measured_traffic(B, dom),
+                     "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"])},
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         "transforms_per_s": 2 * args.steps / elapsed,
         "roundtrip_rel_l2": err,
