@@ -349,6 +349,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--fft-chunk", type=int, default=0, help="beta-slices per FFT chunk (0 = automatic)")
+    ap.add_argument("--dwt-variant", type=int, default=0, help="work shape of the FP64 DWT kernels (0 = default)")
     ap.add_argument("--batch-signals", type=int, default=64,
                     help="signals per batched mixed-precision round trip at the same bandwidth (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -381,6 +382,7 @@ def main():
     ctx = P.Context(0)
     ctx.warm(B)
     ctx.tune("fft_chunk_slices", args.fft_chunk)
+    ctx.tune("dwt_variant", args.dwt_variant)
     coeffs = torch.from_numpy(P.seeded_coeffs(B, args.seed)).cuda()
     grid = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
     back = torch.empty_like(coeffs)

@@ -256,6 +256,7 @@ struct pfs_context {
     long fft_chunk_slices = 0;
     long batch_chunk_signals = 64;
     long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
+    long dwt_variant = 0;       // work shape of the FP64 DWT kernels (2B <= 256)
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -534,7 +535,7 @@ int dsoft_dev(pfs_context* c, int B, const double2* grid, double2* coeffs, hipSt
     double2* G = (double2*)c->work[1];
     rc = fft_fwd(c, *p, grid, W, G, st);
     if (rc) return rc;
-    HIP_TRY(timed(c, "dwt_fwd", st, [&] { return launch_dwt_fwd_f64(p->t, G, coeffs, fwd_norm(B), st); }));
+    HIP_TRY(timed(c, "dwt_fwd", st, [&] { return launch_dwt_fwd_f64(p->t, G, coeffs, fwd_norm(B), st, (int)c->dwt_variant); }));
     return PFS_OK;
 }
 
@@ -548,7 +549,7 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     double2* W = (double2*)c->work[0];
     double2* G = (double2*)c->work[1];
     HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st); }));
-    HIP_TRY(timed(c, "dwt_inv", st, [&] { return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st); }));
+    HIP_TRY(timed(c, "dwt_inv", st, [&] { return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st, (int)c->dwt_variant); }));
     return fft_inv(c, *p, G, W, grid, st);
 }
 
@@ -986,7 +987,8 @@ int pfs_stage_dwt_fwd_dev(pfs_context* c, int B, const void* pencils_dev, void* 
     Plan* p;
     int rc = get_plan(c, B, &p);
     if (rc) return rc;
-    HIP_TRY(launch_dwt_fwd_f64(p->t, (const double2*)pencils_dev, (double2*)coeffs_dev, fwd_norm(B), pick(c, stream)));
+    HIP_TRY(launch_dwt_fwd_f64(p->t, (const double2*)pencils_dev, (double2*)coeffs_dev, fwd_norm(B), pick(c, stream),
+                               (int)c->dwt_variant));
     return PFS_OK;
 }
 
@@ -998,7 +1000,8 @@ int pfs_stage_dwt_inv_dev(pfs_context* c, int B, const void* coeffs_dev, void* p
     Plan* p;
     int rc = get_plan(c, B, &p);
     if (rc) return rc;
-    HIP_TRY(launch_dwt_inv_f64(p->t, (const double2*)coeffs_dev, (double2*)pencils_dev, inv_norm(B), pick(c, stream)));
+    HIP_TRY(launch_dwt_inv_f64(p->t, (const double2*)coeffs_dev, (double2*)pencils_dev, inv_norm(B), pick(c, stream),
+                               (int)c->dwt_variant));
     return PFS_OK;
 }
 
@@ -1050,6 +1053,11 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->batch_interleave = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "dwt_variant")) {
+        if (value < 0 || value > 3) return fail(PFS_ERR_ARG, "dwt_variant must be in 0..3");
+        c->dwt_variant = value;
+        return PFS_OK;
+    }
     return fail(PFS_ERR_ARG, "pfs_tune_set: unknown key '%s'", key);
 }
 
@@ -1067,6 +1075,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_interleave")) {
         *value = c->batch_interleave;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "dwt_variant")) {
+        *value = c->dwt_variant;
         return PFS_OK;
     }
     return fail(PFS_ERR_ARG, "pfs_tune_get: unknown key '%s'", key);

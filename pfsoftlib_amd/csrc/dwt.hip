@@ -186,6 +186,13 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__
 // 16 rows an MFMA operand read touches fall in distinct banks).
 __host__ __device__ constexpr int tile_stride(int nc) { return 64 * nc + 1; }
 
+// Doubles of LDS per wave of the inverse kernel: its 16-row tile, which after
+// the contraction holds the 64 x OSTR output staging of one chain.
+__host__ __device__ constexpr int inv_wave_region(int nc)
+{
+    return 16 * tile_stride(nc) > 64 * OSTR ? 16 * tile_stride(nc) : 64 * OSTR;
+}
+
 // Per-lane view of one case of the pair (hoisted out of the degree loop).
 struct LaneCase {
     int Mc, Mpc, active;
@@ -314,6 +321,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
 {
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NCH);
+    constexpr int WREG = inv_wave_region(NCH);  // a wave's tile, reused as its output staging
     const int B = t.B, N = t.N;
     const int2 pr = t.pairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
@@ -323,8 +331,8 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int nkb = SMALL ? ((N + 15) >> 4) : 4;  // 16-node blocks per chain
 
-    double* tile = sm + w * 16 * TS;
-    double* cf = sm + WPP * 16 * TS;
+    double* tile = sm + w * WREG;
+    double* cf = sm + WPP * WREG;
 
     const int kw = w * 64 * NCH;
     ChainTables<NCH> ct;
@@ -335,20 +343,22 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     const int j = lane & 15, bsel = lane >> 4;
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
     const double* csrc = reinterpret_cast<const double*>(coeffs) + (j & 1);
-    auto load_b = [&](int chk, double* b) {
+    // Gathers of chunk chk: the raw values and their factor n' * sign (0 past R
+    // and for inactive cases; coefficient 0 stands in for the load).  The
+    // loads are unconditional and scaled only at use: a branch around them, or
+    // an immediate use, makes the compiler wait for each load right away.
+    auto load_b = [&](int chk, double* raw, double* fac) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int r = chk * 16 + 4 * s + bsel;
-            double v = 0.0;
-            if (lc.active && r < R) {
-                const double sg = (r & 1) ? lc.sgn_odd : lc.sgn;
-                v = sg * (norm * csrc[2 * coeff_index32(M + r, lc.Mc, lc.Mpc)]);
-            }
-            b[s] = v;
+            const bool live = lc.active && r < R;
+            const int ci = live ? 2 * coeff_index32(M + r, lc.Mc, lc.Mpc) : 0;
+            fac[s] = live ? norm * ((r & 1) ? lc.sgn_odd : lc.sgn) : 0.0;
+            raw[s] = csrc[ci];
         }
     };
-    double bcur[4], bnext[4];
-    load_b(0, bcur);
+    double rcur[4], fcur[4], rnext[4], fnext[4];
+    load_b(0, rcur, fcur);
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
     Chains<NCH> ch;
@@ -359,10 +369,13 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
 
     for (int chk = 0; chk < nch; ++chk) {
-        if (chk + 1 < nch) load_b(chk + 1, bnext);
+        load_b(chk + 1 < nch ? chk + 1 : chk, rnext, fnext);
         __syncthreads();
         run_chains<NCH, 0, NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
         __syncthreads();
+        double bcur[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bcur[s] = rcur[s] * fcur[s];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -374,7 +387,10 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
                         acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[s], acc[4 * c + kb], 0, 0, 0);
                     }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) bcur[s] = bnext[s];
+        for (int s = 0; s < 4; ++s) {
+            rcur[s] = rnext[s];
+            fcur[s] = fnext[s];
+        }
     }
 
     // stage each chain's 64 x 16 block and write each case's beta-vector run
@@ -387,7 +403,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
         cflip |= co.flip << cs;
         cact |= co.active << cs;
     }
-    double* stage = tile;  // 64 * OSTR <= 16 * TS
+    double* stage = tile;  // 64 rows x OSTR fit in the wave's region
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         __syncthreads();
@@ -416,53 +432,85 @@ struct DwtShape {
     int nch, nh, wpp;
 };
 
-// Grids up to 256 nodes (larger ones go to dwt_large.hip).  Forward: one wave
-// per pair, chains two at a time through a 16 KB tile so two waves fit per
-// SIMD.  Inverse: two chains per lane and one wave per 128 nodes (its waves
-// need no reduction; fewer accumulators keep occupancy up).
-static DwtShape dwt_shape(int N, bool fwd)
+// Grids up to 256 nodes (larger ones go to dwt_large.hip).  Default forward
+// shape: one wave per pair, chains two at a time through a 16 KB tile so two
+// waves fit per SIMD; inverse: two chains per lane and one wave per 128 nodes
+// (its waves need no reduction).  Other variants (tuning knob "dwt_variant",
+// forward and inverse alike) split the nodes over more waves per pair.
+static DwtShape dwt_shape(int N, bool fwd, int variant)
 {
     if (N <= 64) return {1, 1, 1};
-    if (N <= 128) return {2, 2, 1};
-    return fwd ? DwtShape{4, 2, 1} : DwtShape{2, 2, 2};
+    if (N <= 128) return variant == 1 ? DwtShape{1, 1, 2} : DwtShape{2, 2, 1};
+    if (fwd) {
+        switch (variant) {
+        case 1: return {2, 2, 2};
+        case 2: return {2, 1, 2};
+        case 3: return {1, 1, 4};
+        default: return {4, 2, 1};
+        }
+    }
+    switch (variant) {
+    case 1: return {1, 1, 4};
+    case 2: return {4, 4, 1};
+    default: return {2, 2, 2};
+    }
 }
 
-static size_t dwt_smem(int B, int N, bool fwd)
+static size_t dwt_smem(int B, int N, bool fwd, int variant)
 {
-    DwtShape s = dwt_shape(N, fwd);
+    DwtShape s = dwt_shape(N, fwd, variant);
     int RP = ((B + 15) / 16) * 16;
-    size_t d = (size_t)s.wpp * 16 * tile_stride(s.nh) + 3 * RP;
+    size_t d = (size_t)s.wpp * (fwd ? 16 * tile_stride(s.nh) : inv_wave_region(s.nch)) + 3 * RP;
     if (fwd && s.wpp > 1) d += (size_t)s.wpp * 256;
     return d * sizeof(double);
 }
 
-hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st)
+hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st,
+                              int variant)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
     if (t.N > 256) return launch_dwt_fwd_f64_large(t, G, coeffs, norm, st);
-    DwtShape s = dwt_shape(t.N, true);
-    size_t smem = dwt_smem(t.B, t.N, true);
+    const DwtShape s = dwt_shape(t.N, true, variant);
+    const size_t smem = dwt_smem(t.B, t.N, true, variant);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     dim3 grid(t.npairs), block(64 * s.wpp);
+#define PFS_FWD(NC, NH_, W)                                                                                 \
+    else if (s.nch == NC && s.nh == NH_ && s.wpp == W)                                                      \
+        hipLaunchKernelGGL((k_dwt_fwd_f64<NC, NH_, W, false>), grid, block, smem, st, t, G, coeffs, norm);
     if (t.N < 64) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_fwd_f64<1, 1, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.nch == 2) hipLaunchKernelGGL((k_dwt_fwd_f64<2, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else hipLaunchKernelGGL((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    PFS_FWD(1, 1, 1)
+    PFS_FWD(2, 2, 1)
+    PFS_FWD(4, 2, 1)
+    PFS_FWD(1, 1, 2)
+    PFS_FWD(2, 2, 2)
+    PFS_FWD(2, 1, 2)
+    PFS_FWD(1, 1, 4)
+    else return hipErrorInvalidValue;
+#undef PFS_FWD
     return hipGetLastError();
 }
 
-hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st)
+hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st,
+                              int variant)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
     if (t.N > 256) return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
-    DwtShape s = dwt_shape(t.N, false);
-    size_t smem = dwt_smem(t.B, t.N, false);
+    const DwtShape s = dwt_shape(t.N, false, variant);
+    const size_t smem = dwt_smem(t.B, t.N, false, variant);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     dim3 grid(t.npairs), block(64 * s.wpp);
+#define PFS_INV(NC, W)                                                                                      \
+    else if (s.nch == NC && s.wpp == W)                                                                     \
+        hipLaunchKernelGGL((k_dwt_inv_f64<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm);
     if (t.N < 64) hipLaunchKernelGGL((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.nch == 1) hipLaunchKernelGGL((k_dwt_inv_f64<1, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.wpp == 1) hipLaunchKernelGGL((k_dwt_inv_f64<2, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else hipLaunchKernelGGL((k_dwt_inv_f64<2, 2, false>), grid, block, smem, st, t, coeffs, G, norm);
+    PFS_INV(1, 1)
+    PFS_INV(2, 1)
+    PFS_INV(2, 2)
+    PFS_INV(1, 2)
+    PFS_INV(1, 4)
+    PFS_INV(4, 1)
+    else return hipErrorInvalidValue;
+#undef PFS_INV
     return hipGetLastError();
 }
 

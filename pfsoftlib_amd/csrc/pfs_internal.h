@@ -97,11 +97,13 @@ hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m,
 
 // ---- DWT stage -----------------------------------------------------------
 // Forward: G pencils (FP64) -> packed coefficients, scaled by pi/(2B^2).
+// variant: work shape of the 2B <= 256 kernels (chains per lane x waves per
+// pair); 0 is the default, see dwt.hip.
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm,
-                              hipStream_t st);
+                              hipStream_t st, int variant = 0);
 // Inverse: packed coefficients -> G pencils (entries with p == B or q == B untouched).
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm,
-                              hipStream_t st);
+                              hipStream_t st, int variant = 0);
 
 // Variants for 2B >= 512 (dwt_large.hip).
 hipError_t launch_dwt_fwd_f64_large(const PlanTables& t, const double2* G, double2* coeffs, double norm,

@@ -125,6 +125,20 @@ def test_dsoft_b128_matches_oracle(ctx, b128):
     assert np.abs(got.real - c.real).max() <= 1e-11 and np.abs(got.imag - c.imag).max() <= 1e-11
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
+    # every work shape of the FP64 DWT kernels (chains per lane x waves per pair)
+    B, c, g_ref, c_ref = b128
+    ctx.tune("dwt_variant", variant)
+    try:
+        assert rel(ctx.idsoft(B, c), g_ref) < TOL
+        assert rel(ctx.dsoft(B, g_ref), c_ref) < TOL
+        c64 = O.seeded_coeffs(64, 77)
+        assert rel(ctx.idsoft(64, c64), O.idsoft(64, c64)) < TOL
+    finally:
+        ctx.tune("dwt_variant", 0)
+
+
 @pytest.mark.parametrize("B", [4, 32])
 def test_idsoft_reads_stale_nyquist_like_reference(ctx, B):
     N = 2 * B
