@@ -105,6 +105,17 @@ int pfs_dsoft_batch_f32_dev(pfs_context* ctx, int bandwidth, int nsig, const voi
 int pfs_idsoft_batch_f32_dev(pfs_context* ctx, int bandwidth, int nsig, const void* coeffs_dev, void* grids_dev,
                              void* stream);
 
+/* Autotuner of the batched path (no counterpart in the reference): times every
+ * work shape of the two Wigner contraction kernels (waves per workgroup x
+ * signals per wave) for batches of 1, 2, 4, ... max_signals signals at this
+ * bandwidth and keeps the fastest per direction and size bucket in the
+ * context; later batches of a covered size use it, others a fixed rule.
+ * Results do not depend on the shape.  pfs_batch_shape reports the shape a
+ * batch of nsig signals would use. */
+int pfs_batch_autotune(pfs_context* ctx, int bandwidth, int max_signals);
+int pfs_batch_shape(pfs_context* ctx, int bandwidth, int forward, int nsig, int* waves_per_block,
+                    int* signals_per_wave);
+
 /* Layer-wise unnormalised 2D DFT of lays square slices, in place, host
  * memory.  Replaces uzl_fftw_layer_wise_DFT2_grid3D / _IDFT2_grid3D
  * (include/PFSOFTlib_headers/fftw_wrapper.hpp:28-33, src/fftw_wrapper.cpp:47-172):
@@ -143,6 +154,11 @@ int pfs_stage_dwt_inv_dev(pfs_context* ctx, int bandwidth, const void* coeffs_de
 int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_dev, void* grid_dev, void* stream);
 
 /* Tuning knobs (no counterpart in the reference).  Keys:
+ *   "batch_interleave"  1 (default): batched contraction on signal-interleaved
+ *                       coefficients with transposes; 0: on the caller's layout.
+ *   "dwt_variant"       work shape of the FP64 DWT kernels for 2B <= 256
+ *                       (0 default, 1..3 alternatives; results equal).
+ *   "server_graphs"     see the request server below.
  *   "fft_chunk_slices"  beta-slices per FFT chunk (0 = automatic: one chunk);
  *                       rounded to a power of two dividing 2B.  Smaller
  *                       chunks keep the row-pass output in the Infinity
@@ -182,10 +198,12 @@ int pfs_synchronize(pfs_context* ctx, void* stream);
  * hbm_budget_bytes (0 = only max_batch limits) bounds the server's own device
  * memory: resident FP32 tables of the served bandwidths plus the slots' work
  * buffers (two complex64 grids per signal of a batch).  A result does not
- * depend on which requests share its batch.  The server's streams are ordered
- * after work enqueued on the HIP null stream before a submission; work on
- * other streams that writes a request's input must be complete when it is
- * submitted.  Submit and collect/wait may be called from any thread; a ticket
+ * depend on which requests share its batch.  Each batch is ordered after the
+ * work enqueued on the HIP null stream before it starts; work on other streams
+ * that writes a request's input must be complete when it is submitted.  The
+ * launch sequence of each (direction, bandwidth, batch size) is captured once
+ * per slot into a HIP graph and replayed (context tune key "server_graphs",
+ * read at creation: 1 default, 0 = launch kernels directly).  Submit and collect/wait may be called from any thread; a ticket
  * is consumed by pfs_server_wait or by pfs_server_collect, not both. */
 typedef struct pfs_server pfs_server;
 
@@ -202,6 +220,7 @@ typedef struct pfs_server_stats_t {
     int max_inflight_batches;      /* most batches on the device at once    */
     size_t device_bytes;           /* tables + work buffers held            */
     size_t slot_bytes;             /* work-buffer bytes per slot            */
+    uint64_t graphs;               /* batch graphs captured (replayed after) */
 } pfs_server_stats_t;
 
 int pfs_server_create(pfs_context* ctx, const int* bandwidths, int nbw, int slots, int max_batch,

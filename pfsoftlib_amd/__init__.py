@@ -36,7 +36,7 @@ PFS_REQ_INVERSE = 1
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("signals", ctypes.c_uint64),
                 ("pending", ctypes.c_uint64), ("max_inflight_batches", ctypes.c_int),
-                ("device_bytes", ctypes.c_size_t), ("slot_bytes", ctypes.c_size_t)]
+                ("device_bytes", ctypes.c_size_t), ("slot_bytes", ctypes.c_size_t), ("graphs", ctypes.c_uint64)]
 
 
 class PfsError(RuntimeError):
@@ -85,6 +85,8 @@ def lib():
             "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_memcpy_d2h": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_synchronize": (ip, [vp, vp]),
+            "pfs_batch_autotune": (ip, [vp, ip, ip]),
+            "pfs_batch_shape": (ip, [vp, ip, ip, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]),
             "pfs_server_create": (ip, [vp, ctypes.POINTER(ip), ip, ip, ip, ctypes.c_size_t, ctypes.POINTER(vp)]),
             "pfs_server_destroy": (ip, [vp]),
             "pfs_server_submit": (ip, [vp, ip, ip, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
@@ -218,6 +220,16 @@ class Context:
     def idsoft_batch_f32(self, B, nsig, coeffs_dev, grids_dev, stream=None):
         _check(lib().pfs_idsoft_batch_f32_dev(self._h, B, nsig, _devptr(coeffs_dev), _devptr(grids_dev),
                                               _stream_handle(stream)))
+
+    def autotune_batch(self, B, max_signals):
+        """Times the batched contraction shapes for 1, 2, 4, ... max_signals signals."""
+        _check(lib().pfs_batch_autotune(self._h, B, max_signals))
+
+    def batch_shape(self, B, forward, nsig):
+        """(waves per workgroup, signals per wave) a batch of nsig signals uses."""
+        w, s = ctypes.c_int(), ctypes.c_int()
+        _check(lib().pfs_batch_shape(self._h, B, 1 if forward else 0, nsig, ctypes.byref(w), ctypes.byref(s)))
+        return w.value, s.value
 
     def stage_fft_fwd(self, B, grid_dev, pencils_dev, stream=None):
         _check(lib().pfs_stage_fft_fwd_dev(self._h, B, _devptr(grid_dev), _devptr(pencils_dev), _stream_handle(stream)))

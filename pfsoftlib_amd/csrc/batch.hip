@@ -54,10 +54,11 @@ __device__ __forceinline__ BCase bcase(int c, int M, int Mp, int N)
     return ci;
 }
 
-template <int LAYOUT>
-__device__ __forceinline__ int out_row(int lane, int v)
+// Output row of accumulator register v in lane `lane` for the FP32 MFMA
+// output layout the context's self-test found (0 or 1).
+__device__ __forceinline__ int out_row(int lane, int v, int layout)
 {
-    return LAYOUT == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v;
+    return layout == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v;
 }
 
 // FP32 tables for pair index pi, rows r < RP(pi) (16-padded, zero beyond R),
@@ -194,10 +195,10 @@ __device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN,
         }
 }
 
-template <int NN, int SPW, int NC, int LAYOUT>
+template <int NN, int SPW, int NC>
 __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const float* tf, const float* wl, int e0,
                                          int de, float2* __restrict__ coeffs, const BCase& cb, int M, int R, int s0,
-                                         int nsig, long cdist, long cstride, float norm, int lane)
+                                         int nsig, long cdist, long cstride, float norm, int lane, int layout)
 {
     constexpr int NKB = NN / 16;
     v4f acc[NC][SPW];
@@ -238,7 +239,7 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
         for (int ch = 0; ch < NC; ++ch)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = ch * 16 + out_row<LAYOUT>(lane, v);
+                const int r = ch * 16 + out_row(lane, v, layout);
                 if (r < R) {
                     const float sg = (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
                     cs[2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc)] = norm * (sg * acc[ch][sp][v]);
@@ -248,26 +249,27 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
 }
 
 // Run fwd_pair with the pair's chunk count as a template argument.
-template <int NN, int SPW, int LAYOUT, int NC>
+template <int NN, int SPW, int NC>
 __device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[SPW], const float* tf, const float* wl,
                                              int e0, int de, float2* __restrict__ coeffs, const BCase& cb, int M, int R,
-                                             int s0, int nsig, long cdist, long cstride, float norm, int lane)
+                                             int s0, int nsig, long cdist, long cstride, float norm, int lane, int layout)
 {
     if constexpr (NC <= (NN / 2 + 15) / 16) {
         if (nch == NC)
-            fwd_pair<NN, SPW, NC, LAYOUT>(gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane);
+            fwd_pair<NN, SPW, NC>(gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane, layout);
         else
-            fwd_dispatch<NN, SPW, LAYOUT, NC + 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride,
-                                                  norm, lane);
+            fwd_dispatch<NN, SPW, NC + 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
+                                          lane, layout);
     }
 }
 
-template <int NN, int WPB, int SPW, int LAYOUT>
+template <int NN, int WPB, int SPW>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
                                                                 const float* __restrict__ wts32,
                                                                 const float* __restrict__ Tf,
                                                                 const float2* __restrict__ G, float2* __restrict__ coeffs,
-                                                                int nsig, int nsg, long cdist, long cstride, float norm)
+                                                                int nsig, int nsg, long cdist, long cstride, float norm,
+                                                                int layout)
 {
     constexpr int NKB = NN / 16;
     const BlockWork bw = block_work(nsg);
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
     const int e0 = 2 * (cb.flip ? NN - 4 - 4 * kk + 2 * (1 - p) : 4 * kk + 2 * p);
     const int de = cb.flip ? -32 : 32;
 
-    fwd_dispatch<NN, SPW, LAYOUT, 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane);
+    fwd_dispatch<NN, SPW, 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane, layout);
 }
 
 constexpr int OSTRF = 17;  // output staging row stride (floats), NN rows of 16
@@ -350,11 +352,12 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
 // A = column-major table tiles (k rows, r columns), B = the coefficients,
 // gathered per lane.  The 8 beta-vectors of each signal are staged per wave
 // in LDS and written out as contiguous pencils.
-template <int NN, int WPB, int SPW, int LAYOUT>
+template <int NN, int WPB, int SPW>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, const long* __restrict__ roff,
                                                                 const float* __restrict__ Ti,
                                                                 const float2* __restrict__ coeffs, float2* __restrict__ G,
-                                                                int nsig, int nsg, long cdist, long cstride, float norm)
+                                                                int nsig, int nsg, long cdist, long cstride, float norm,
+                                                                int layout)
 {
     constexpr int NKB = NN / 16;
     extern __shared__ __align__(16) float smf[];
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sw[(16 * kb + out_row<LAYOUT>(lane, v)) * OSTRF + i] = acc[kb][sp][v];
+            for (int v = 0; v < 4; ++v) sw[(16 * kb + out_row(lane, v, layout)) * OSTRF + i] = acc[kb][sp][v];
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
         for (int c = 0; c < 8; ++c) {
@@ -475,47 +478,63 @@ __global__ void k_nyquist_zero_f32(float2* __restrict__ G, int B, int nsig)
     G[s * (long)N * N * N + ((long)p * N + q) * N + k] = make_float2(0.f, 0.f);
 }
 
-template <int NN> struct BatchShape;  // waves per block, signals per wave (forward, inverse)
-template <> struct BatchShape<16> { static constexpr int WPB = 4, SPWF = 4, SPWI = 4; };
-template <> struct BatchShape<32> { static constexpr int WPB = 4, SPWF = 4, SPWI = 4; };
-template <> struct BatchShape<64> { static constexpr int WPB = 4, SPWF = 4, SPWI = 4; };
-template <> struct BatchShape<128> { static constexpr int WPB = 4, SPWF = 4, SPWI = 2; };
-template <> struct BatchShape<256> { static constexpr int WPB = 4, SPWF = 2, SPWI = 2; };
-
 inline dim3 batch_grid(int npairs, int nsg) { return dim3((unsigned)(8 * ((npairs + 7) / 8) * nsg)); }
+
+// Signals per wave a kernel instance may hold: at 2B = 256 four signals of
+// accumulators (8 chunks x 4 x 4 forward, 16 blocks x 4 x 4 inverse) would
+// leave one wave per SIMD.
+constexpr int max_spw(int NN, bool fwd) { return NN >= 256 ? 2 : (fwd || NN <= 64 ? 4 : 2); }
+
+template <int NN, int WPB, int SPW>
+hipError_t fwd_launch(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long cdist,
+                      long cstride, float norm, hipStream_t st)
+{
+    if constexpr (SPW > max_spw(NN, true)) {
+        return hipErrorInvalidValue;
+    } else {
+        const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
+        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), 0, st, t,
+                           bt.roff, bt.wts32, bt.tf, G, coeffs, nsig, nsg, cdist, cstride, norm, g_mfma_f32_layout);
+        return hipGetLastError();
+    }
+}
+
+template <int NN, int WPB, int SPW>
+hipError_t inv_launch(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
+                      long cstride, float norm, hipStream_t st)
+{
+    if constexpr (SPW > max_spw(NN, false)) {
+        return hipErrorInvalidValue;
+    } else {
+        const size_t smem = sizeof(float) * (size_t)WPB * NN * OSTRF;
+        if (smem > lds_limit()) return hipErrorInvalidConfiguration;
+        const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
+        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), smem, st, t,
+                           bt.roff, bt.ti, coeffs, G, nsig, nsg, cdist, cstride, norm, g_mfma_f32_layout);
+        return hipGetLastError();
+    }
+}
 
 template <int NN>
 hipError_t fwd_nn(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long cdist,
-                  long cstride, float norm, hipStream_t st)
+                  long cstride, float norm, BatchKernelShape sh, hipStream_t st)
 {
-    constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPWF;
-    const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
-    const dim3 grid = batch_grid(t.npairs, nsg), block(64 * WPB);
-    if (g_mfma_f32_layout == 0)
-        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 0>), grid, block, 0, st, t, bt.roff, bt.wts32, bt.tf, G,
-                           coeffs, nsig, nsg, cdist, cstride, norm);
-    else
-        hipLaunchKernelGGL((k_dwt_fwd_batch_f32<NN, WPB, SPW, 1>), grid, block, 0, st, t, bt.roff, bt.wts32, bt.tf, G,
-                           coeffs, nsig, nsg, cdist, cstride, norm);
-    return hipGetLastError();
+#define PFS_B(W, S) \
+    if (sh.wpb == W && sh.spw == S) return fwd_launch<NN, W, S>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    PFS_B(1, 1) PFS_B(1, 2) PFS_B(1, 4) PFS_B(2, 1) PFS_B(2, 2) PFS_B(2, 4) PFS_B(4, 1) PFS_B(4, 2) PFS_B(4, 4)
+#undef PFS_B
+    return hipErrorInvalidValue;
 }
 
 template <int NN>
 hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
-                  long cstride, float norm, hipStream_t st)
+                  long cstride, float norm, BatchKernelShape sh, hipStream_t st)
 {
-    constexpr int WPB = BatchShape<NN>::WPB, SPW = BatchShape<NN>::SPWI;
-    const size_t smem = sizeof(float) * (size_t)WPB * NN * OSTRF;
-    if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
-    const dim3 grid = batch_grid(t.npairs, nsg), block(64 * WPB);
-    if (g_mfma_f32_layout == 0)
-        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 0>), grid, block, smem, st, t, bt.roff, bt.ti, coeffs, G,
-                           nsig, nsg, cdist, cstride, norm);
-    else
-        hipLaunchKernelGGL((k_dwt_inv_batch_f32<NN, WPB, SPW, 1>), grid, block, smem, st, t, bt.roff, bt.ti, coeffs, G,
-                           nsig, nsg, cdist, cstride, norm);
-    return hipGetLastError();
+#define PFS_B(W, S) \
+    if (sh.wpb == W && sh.spw == S) return inv_launch<NN, W, S>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    PFS_B(1, 1) PFS_B(1, 2) PFS_B(1, 4) PFS_B(2, 1) PFS_B(2, 2) PFS_B(2, 4) PFS_B(4, 1) PFS_B(4, 2) PFS_B(4, 4)
+#undef PFS_B
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -536,28 +555,48 @@ hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, h
     return hipGetLastError();
 }
 
-hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
-                                    int nsig, long cdist, long cstride, float norm, hipStream_t st)
+BatchKernelShape batch_default_shape(int N, bool fwd, int n)
 {
+    // the full shape (4 waves of the most signals a wave may hold) once the batch
+    // fills it; fewer signals per wave and fewer waves for small batches, so no
+    // wave idles and no wave repeats a clamped signal
+    const int smax = N >= 256 ? 2 : (fwd || N <= 64 ? 4 : 2);
+    const int spw = n >= smax ? smax : (n >= 2 ? 2 : 1);
+    const int per = (n + spw - 1) / spw;
+    return {per >= 4 ? 4 : (per >= 2 ? 2 : 1), spw};
+}
+
+bool batch_shape_valid(int N, bool fwd, BatchKernelShape sh)
+{
+    const int smax = N >= 256 ? 2 : (fwd || N <= 64 ? 4 : 2);
+    return (sh.wpb == 1 || sh.wpb == 2 || sh.wpb == 4) && (sh.spw == 1 || sh.spw == 2 || sh.spw == 4) &&
+           sh.spw <= smax;
+}
+
+hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
+                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh, hipStream_t st)
+{
+    if (sh.wpb == 0) sh = batch_default_shape(t.N, true, nsig);
     switch (t.N) {
-    case 16: return fwd_nn<16>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
-    case 32: return fwd_nn<32>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
-    case 64: return fwd_nn<64>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
-    case 128: return fwd_nn<128>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
-    case 256: return fwd_nn<256>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    case 16: return fwd_nn<16>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 32: return fwd_nn<32>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 64: return fwd_nn<64>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 128: return fwd_nn<128>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 256: return fwd_nn<256>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
-                                    int nsig, long cdist, long cstride, float norm, hipStream_t st)
+                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh, hipStream_t st)
 {
+    if (sh.wpb == 0) sh = batch_default_shape(t.N, false, nsig);
     switch (t.N) {
-    case 16: return inv_nn<16>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
-    case 32: return inv_nn<32>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
-    case 64: return inv_nn<64>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
-    case 128: return inv_nn<128>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
-    case 256: return inv_nn<256>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    case 16: return inv_nn<16>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 32: return inv_nn<32>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 64: return inv_nn<64>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 128: return inv_nn<128>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 256: return inv_nn<256>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -225,6 +225,9 @@ struct Plan {
     void* d32 = nullptr;  // row offsets, FP32 weights and both tile tables
     size_t d32_bytes = 0;
     BatchTables bt{};
+    // batched contraction shapes picked by the autotuner per direction and
+    // batch-size bucket (ceil(log2 n), capped at 7); {0, 0} = default rule
+    BatchKernelShape tuned[2][8] = {};
 };
 
 }  // namespace
@@ -257,6 +260,9 @@ struct pfs_context {
     long batch_chunk_signals = 64;
     long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
     long dwt_variant = 0;       // work shape of the FP64 DWT kernels (2B <= 256)
+    long server_graphs = 1;     // request servers replay captured HIP graphs
+    long server_autotune = 1;   // request servers autotune the batched shapes at creation
+    long batch_force_shape = 0; // 0: tuned / default shapes; else 10 * waves + signals per wave
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -649,6 +655,24 @@ int batch_prepare(pfs_context* c, int B, int nsig, Plan** pp, int* sb, hipStream
     return ensure_bwork(c, (size_t)*sb * N * N * N * sizeof(float2));
 }
 
+int size_bucket(int n)
+{
+    int b = 0;
+    while ((1 << b) < n && b < 7) ++b;
+    return b;
+}
+
+// Contraction shape for a batch of n signals: the autotuned one for n's size
+// bucket when the autotuner has covered it, else the default rule.
+BatchKernelShape shape_for(const pfs_context* c, const Plan& p, bool fwd, int n)
+{
+    if (c->batch_force_shape) {
+        const BatchKernelShape sh{(int)c->batch_force_shape / 10, (int)c->batch_force_shape % 10};
+        if (batch_shape_valid(p.t.N, fwd, sh)) return sh;  // else the rule below
+    }
+    return p.tuned[fwd ? 1 : 0][size_bucket(n)];  // {0, 0} until the autotuner covered n's bucket
+}
+
 // Row length of the interleaved coefficient layout for a sub-batch of n
 // signals in work buffers sized for cap signals: padded to 8 signals (32-byte
 // runs) when the buffer has room.
@@ -656,6 +680,59 @@ int interleave_ld(int n, int cap, long count, long N3)
 {
     const int ld = (n + 7) & ~7;
     return (long)ld * count <= (long)cap * N3 ? ld : n;
+}
+
+// Times every valid contraction shape on work buffers of n signals and keeps
+// the fastest per direction and size bucket (n = 1, 2, 4, ... up to max_n).
+int batch_autotune(pfs_context* c, int B, int max_n)
+{
+    Plan* p;
+    int sb;
+    int rc = batch_prepare(c, B, max_n, &p, &sb, c->stream);
+    if (rc) return rc;
+    const int N = p->t.N;
+    const long N3 = (long)N * N * N, count = coeff_offset(B);
+    float2* W = (float2*)c->bwork[0];
+    float2* G = (float2*)c->bwork[1];
+    const float fnorm = (float)fwd_norm(B), inorm = (float)inv_norm(B);
+    hipEvent_t e0 = take_event(c), e1 = take_event(c);
+    if (!e0 || !e1) return fail(PFS_ERR_DEVICE, "batch_autotune: event creation failed");
+    // defined inputs: zero pencils and coefficients (timing does not depend on values)
+    HIP_TRY(hipMemsetAsync(W, 0, (size_t)sb * N3 * sizeof(float2), c->stream));
+    HIP_TRY(hipMemsetAsync(G, 0, (size_t)sb * N3 * sizeof(float2), c->stream));
+    for (int n = 1; n <= sb; n *= 2) {
+        const int ld = interleave_ld(n, sb, count, N3);
+        for (int dir = 0; dir < 2; ++dir) {
+            float best = 1e30f;
+            BatchKernelShape pick{0, 0};
+            for (int w : {1, 2, 4})
+                for (int sp : {1, 2, 4}) {
+                    const BatchKernelShape sh{w, sp};
+                    if (!batch_shape_valid(N, dir == 1, sh)) continue;
+                    float t = 1e30f;
+                    for (int rep = 0; rep < 3; ++rep) {
+                        HIP_TRY(hipEventRecord(e0, c->stream));
+                        if (dir == 1)
+                            HIP_TRY(launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, fnorm, sh, c->stream));
+                        else
+                            HIP_TRY(launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, inorm, sh, c->stream));
+                        HIP_TRY(hipEventRecord(e1, c->stream));
+                        HIP_TRY(hipEventSynchronize(e1));
+                        float ms = 0;
+                        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                        t = std::min(t, ms);
+                    }
+                    if (t < best) {
+                        best = t;
+                        pick = sh;
+                    }
+                }
+            p->tuned[dir][size_bucket(n)] = pick;
+        }
+    }
+    c->ev_pool.push_back(e0);
+    c->ev_pool.push_back(e1);
+    return PFS_OK;
 }
 
 // Forward batch of n <= cap signals: FFT rows (grids -> W), FFT columns
@@ -675,10 +752,10 @@ int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSe
     if (c->batch_interleave || coeffs.ptrs) {
         // W is free once the column pass has run: the coefficients go there interleaved
         const int ld = interleave_ld(n, cap, count, N3);
-        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, st); }));
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, shape_for(c, *p, true, n), st); }));
         HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs.base, coeffs.ptrs, W, count, n, ld, false, st); }));
     } else {
-        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs.base, n, coeffs.stride, 1, norm, st); }));
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs.base, n, coeffs.stride, 1, norm, shape_for(c, *p, true, n), st); }));
     }
     return PFS_OK;
 }
@@ -700,9 +777,9 @@ int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, Sig
     if (c->batch_interleave || coeffs.ptrs) {
         const int ld = interleave_ld(n, cap, count, N3);
         HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs.base, coeffs.ptrs, W, count, n, ld, true, st); }));
-        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, shape_for(c, *p, false, n), st); }));
     } else {
-        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, shape_for(c, *p, false, n), st); }));
     }
     HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
     HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids.base, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
@@ -1058,6 +1135,23 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->dwt_variant = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "server_graphs")) {
+        if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "server_graphs must be 0 or 1");
+        c->server_graphs = value;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "server_autotune")) {
+        if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "server_autotune must be 0 or 1");
+        c->server_autotune = value;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_force_shape")) {
+        const BatchKernelShape sh{(int)value / 10, (int)value % 10};
+        if (value != 0 && !(batch_shape_valid(256, true, sh) || batch_shape_valid(64, true, sh)))
+            return fail(PFS_ERR_ARG, "batch_force_shape must be 0 or 10 * waves + signals per wave (1/2/4 each)");
+        c->batch_force_shape = value;
+        return PFS_OK;
+    }
     return fail(PFS_ERR_ARG, "pfs_tune_set: unknown key '%s'", key);
 }
 
@@ -1079,6 +1173,18 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "dwt_variant")) {
         *value = c->dwt_variant;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "server_graphs")) {
+        *value = c->server_graphs;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "server_autotune")) {
+        *value = c->server_autotune;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_force_shape")) {
+        *value = c->batch_force_shape;
         return PFS_OK;
     }
     return fail(PFS_ERR_ARG, "pfs_tune_get: unknown key '%s'", key);
@@ -1204,7 +1310,13 @@ int batch_run(pfs_context* c, bool forward, int B, int n, int cap, SigSet in, Si
     auto it = c->plans.find(B);
     if (it == c->plans.end() || !it->second.d32) return fail(PFS_ERR_ARG, "batch_run: bandwidth %d not warmed", B);
     Plan* p = &it->second;
-    return forward ? dsoft_batch_run(c, p, n, cap, in, out, W, G, st) : idsoft_batch_run(c, p, n, cap, in, out, W, G, st);
+    // no timing events here: the server may be capturing this sequence into a graph
+    const bool timing = c->timing;
+    c->timing = false;
+    const int rc = forward ? dsoft_batch_run(c, p, n, cap, in, out, W, G, st)
+                           : idsoft_batch_run(c, p, n, cap, in, out, W, G, st);
+    c->timing = timing;
+    return rc;
 }
 
 }  // namespace pfs
@@ -1220,5 +1332,56 @@ const char* error_text() { return g_last_error.c_str(); }
 
 int context_device(const pfs_context* c) { return c->device; }
 
+bool batch_is_tuned(pfs_context* c, int B, int upto)
+{
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->plans.find(B);
+    if (it == c->plans.end()) return false;
+    int b = 0;
+    while ((1 << (b + 1)) <= upto && b < 7) ++b;  // largest power-of-two bucket <= upto
+    return it->second.tuned[0][b].wpb && it->second.tuned[1][b].wpb;
+}
+
 }  // namespace pfs
+
+extern "C" {
+
+int pfs_batch_autotune(pfs_context* c, int B, int max_signals)
+{
+    if (!c || max_signals < 1) return fail(PFS_ERR_ARG, "pfs_batch_autotune: bad argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    const long keep = c->batch_chunk_signals;
+    const size_t had = c->bwork_bytes;
+    c->batch_chunk_signals = std::max<long>(keep, max_signals);
+    const long force = c->batch_force_shape;
+    c->batch_force_shape = 0;
+    const int rc = batch_autotune(c, B, max_signals);
+    c->batch_force_shape = force;
+    c->batch_chunk_signals = keep;
+    if (c->bwork_bytes > had) {
+        // give back what tuning grew; the next batched call sizes them again
+        (void)hipStreamSynchronize(c->stream);
+        ctx_free(c, c->bwork[0], c->bwork_bytes);
+        ctx_free(c, c->bwork[1], c->bwork_bytes);
+        c->bwork[0] = c->bwork[1] = nullptr;
+        c->bwork_bytes = 0;
+    }
+    return rc;
+}
+
+int pfs_batch_shape(pfs_context* c, int B, int forward, int nsig, int* waves_per_block, int* signals_per_wave)
+{
+    if (!c || !waves_per_block || !signals_per_wave || nsig < 1) return fail(PFS_ERR_ARG, "pfs_batch_shape: bad argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->plans.find(B);
+    BatchKernelShape sh{0, 0};
+    if (it != c->plans.end()) sh = shape_for(c, it->second, forward != 0, nsig);
+    if (!sh.wpb) sh = batch_default_shape(2 * B, forward != 0, nsig);
+    *waves_per_block = sh.wpb;
+    *signals_per_wave = sh.spw;
+    return PFS_OK;
+}
+
+}  // extern "C"
 

@@ -123,12 +123,22 @@ struct BatchTables {
 // Number of table rows (16-padded per pair) for bandwidth B.
 long batch_table_rows(int B);
 hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st);
+// Work shape of a batched contraction launch: waves per workgroup (1, 2, 4) and
+// signals per wave (1, 2, 4; at most 2 at 2B = 256 and for the inverse at
+// 2B = 128).  {0, 0} = the default for the batch size.
+struct BatchKernelShape {
+    int wpb, spw;
+};
+BatchKernelShape batch_default_shape(int N, bool fwd, int nsig);
+bool batch_shape_valid(int N, bool fwd, BatchKernelShape sh);
 // Coefficient (s, idx) of the batch sits at s * cdist + idx * cstride (complex
 // elements): (count, 1) is the per-signal layout, (1, ld) the interleaved one.
 hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
-                                    int nsig, long cdist, long cstride, float norm, hipStream_t st);
+                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh,
+                                    hipStream_t st);
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
-                                    int nsig, long cdist, long cstride, float norm, hipStream_t st);
+                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh,
+                                    hipStream_t st);
 // Between the per-signal layout (per_signal + s * count, or sptr[s] when sptr
 // is set) and the interleaved layout [idx][ld], in the direction given.
 hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float2* interleaved, long count, int n,
@@ -160,5 +170,8 @@ int set_error(int code, const char* msg);
 const char* error_text();
 // Device ordinal of a context.
 int context_device(const pfs_context* c);
+// Whether the batched-shape autotuner has covered batches of up to `upto`
+// signals at bandwidth B on this context.
+bool batch_is_tuned(pfs_context* c, int B, int upto);
 }  // namespace pfs
 

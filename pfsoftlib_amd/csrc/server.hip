@@ -10,6 +10,14 @@
 // the slot's capacity for that bandwidth, and launches them as one batch.
 // Requests that arrive while others run are admitted as slots free up.
 //
+// Launch: the first batch of each (direction, bandwidth, size) on a slot is
+// captured into a HIP graph (pointer-table upload + pipeline kernels), later
+// ones replay it -- one launch instead of eight, which is what bounds the
+// dispatcher on small-bandwidth batches.  Slot streams do not block on the
+// null stream; each batch instead waits on an event recorded on the null
+// stream at launch, so inputs written there before submission are ordered
+// before the batch.  The capture holds no memset nodes.
+//
 // HBM admission: the server's own device memory -- the resident FP32 tables
 // of every served bandwidth plus the slots' work buffers -- stays within the
 // budget given at creation; a slot holds work buffers of budget_work / slots
@@ -52,6 +60,8 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t host_order = nullptr;  // recorded on the null stream before each batch
+    std::unordered_map<uint64_t, hipGraphExec_t> graphs;  // (kind, B, n) -> replayable batch
     float2* work[2] = {nullptr, nullptr};
     float2** host_ptrs = nullptr;  // pinned: 2 * max_batch device addresses
     float2** dev_ptrs = nullptr;
@@ -65,6 +75,7 @@ struct pfs_server {
     pfs_context* ctx = nullptr;
     int device = 0;
     int max_batch = 0;
+    bool use_graphs = true;
     size_t slot_bytes = 0;
     size_t device_bytes = 0;  // tables + work buffers
     std::vector<int> bandwidths;
@@ -81,7 +92,7 @@ struct pfs_server {
     std::string error;
 
     // statistics
-    uint64_t n_batches = 0, n_done = 0, n_signals = 0;
+    uint64_t n_batches = 0, n_done = 0, n_signals = 0, n_graphs = 0;
     int max_inflight = 0;
 };
 
@@ -145,15 +156,55 @@ bool launch(pfs_server* s, Slot& sl)
         sl.host_ptrs[s->max_batch + i] = (float2*)r.out;
     }
     int rc = PFS_OK;
-    hipError_t e = hipMemcpyAsync(sl.dev_ptrs, sl.host_ptrs, 2 * s->max_batch * sizeof(float2*), hipMemcpyHostToDevice,
-                                  sl.stream);
-    if (e != hipSuccess) {
-        rc = PFS_ERR_DEVICE;
-        s->error = std::string("pointer table upload failed: ") + hipGetErrorString(e);
-    } else {
+    hipError_t e = hipSuccess;
+    auto enqueue = [&]() -> int {
+        e = hipMemcpyAsync(sl.dev_ptrs, sl.host_ptrs, 2 * s->max_batch * sizeof(float2*), hipMemcpyHostToDevice,
+                           sl.stream);
+        if (e != hipSuccess) {
+            s->error = std::string("pointer table upload failed: ") + hipGetErrorString(e);
+            return PFS_ERR_DEVICE;
+        }
         const SigSet in{nullptr, 0, sl.dev_ptrs}, out{nullptr, 0, sl.dev_ptrs + s->max_batch};
-        rc = batch_run(s->ctx, kind == PFS_REQ_FORWARD, B, n, cap, in, out, sl.work[0], sl.work[1], sl.stream);
-        if (rc) s->error = error_text();
+        const int r = batch_run(s->ctx, kind == PFS_REQ_FORWARD, B, n, cap, in, out, sl.work[0], sl.work[1], sl.stream);
+        if (r) s->error = error_text();
+        return r;
+    };
+    // order the batch after the null-stream work enqueued before it (inputs)
+    if ((e = hipEventRecord(sl.host_order, nullptr)) != hipSuccess ||
+        (e = hipStreamWaitEvent(sl.stream, sl.host_order, 0)) != hipSuccess) {
+        rc = PFS_ERR_DEVICE;
+        s->error = std::string("ordering after the null stream failed: ") + hipGetErrorString(e);
+    } else if (s->use_graphs) {
+        const uint64_t key = ((uint64_t)kind << 48) | ((uint64_t)B << 24) | (uint64_t)n;
+        auto it = sl.graphs.find(key);
+        if (it == sl.graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            if ((e = hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+                rc = PFS_ERR_DEVICE;
+            } else {
+                rc = enqueue();
+                const hipError_t e2 = hipStreamEndCapture(sl.stream, &g);
+                if (!rc && e2 != hipSuccess) {
+                    e = e2;
+                    rc = PFS_ERR_DEVICE;
+                }
+                if (!rc && (e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess) rc = PFS_ERR_DEVICE;
+                if (g) (void)hipGraphDestroy(g);
+            }
+            if (rc == PFS_ERR_DEVICE && s->error.empty())
+                s->error = std::string("graph capture failed: ") + hipGetErrorString(e);
+            if (!rc) {
+                it = sl.graphs.emplace(key, x).first;
+                ++s->n_graphs;
+            }
+        }
+        if (!rc && (e = hipGraphLaunch(it->second, sl.stream)) != hipSuccess) {
+            rc = PFS_ERR_DEVICE;
+            s->error = std::string("graph launch failed: ") + hipGetErrorString(e);
+        }
+    } else {
+        rc = enqueue();
     }
     if (rc == PFS_OK && (e = hipEventRecord(sl.done, sl.stream)) != hipSuccess) {
         rc = PFS_ERR_DEVICE;
@@ -214,7 +265,10 @@ void release(pfs_server* s)
             if (w) (void)pfs_device_free(s->ctx, w);
         if (sl.dev_ptrs) (void)pfs_device_free(s->ctx, sl.dev_ptrs);
         if (sl.host_ptrs) (void)hipHostFree(sl.host_ptrs);
+        for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
+        sl.graphs.clear();
         if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.host_order) (void)hipEventDestroy(sl.host_order);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     s->slots.clear();
@@ -234,6 +288,9 @@ int pfs_server_create(pfs_context* ctx, const int* bandwidths, int nbw, int slot
     s->ctx = ctx;
     s->device = context_device(ctx);
     s->max_batch = max_batch;
+    long graphs = 1;
+    (void)pfs_tune_get(ctx, "server_graphs", &graphs);
+    s->use_graphs = graphs != 0;
     int rc = PFS_OK;
     if (hipSetDevice(s->device) != hipSuccess) rc = set_error(PFS_ERR_DEVICE, "pfs_server_create: hipSetDevice failed");
     // resident tables of every served bandwidth
@@ -268,14 +325,19 @@ int pfs_server_create(pfs_context* ctx, const int* bandwidths, int nbw, int slot
             }
         s->device_bytes = tables + per_slot * slots;
     }
+    long autotune = 1;
+    (void)pfs_tune_get(ctx, "server_autotune", &autotune);
+    for (size_t i = 0; i < s->bandwidths.size() && !rc && autotune; ++i) {
+        const int B = s->bandwidths[i], cap = cap_for(s, B);
+        if (!batch_is_tuned(ctx, B, cap)) rc = pfs_batch_autotune(ctx, B, cap);
+    }
     for (int i = 0; i < slots && !rc; ++i) {
         s->slots.emplace_back();
         Slot& sl = s->slots.back();
         const size_t half = (s->slot_bytes / 2) & ~(size_t)255;
-        // blocking streams: a batch starts after work the caller put on the null
-        // stream before submitting (e.g. writing the request's input)
-        if (hipStreamCreate(&sl.stream) != hipSuccess ||
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.host_order, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void**)&sl.host_ptrs, 2 * max_batch * sizeof(float2*), hipHostMallocDefault) != hipSuccess) {
             rc = set_error(PFS_ERR_DEVICE, "pfs_server_create: stream, event or pinned buffer creation failed");
             break;
@@ -396,6 +458,7 @@ int pfs_server_stats(pfs_server* s, pfs_server_stats_t* out)
     out->max_inflight_batches = s->max_inflight;
     out->device_bytes = s->device_bytes;
     out->slot_bytes = s->slot_bytes;
+    out->graphs = s->n_graphs;
     return PFS_OK;
 }
 

@@ -103,3 +103,37 @@ def test_batch_interleaved_and_direct_coefficients_agree(ctx, torch, B, nsig, ch
     assert torch.equal(out[0][1], out[1][1])
     gref = O.idsoft(B, cs[:O.coeff_count(B)].astype(np.complex128))
     assert rel(out[1][0][:N ** 3].cpu().numpy().astype(np.complex128), gref) < TOL
+
+
+@pytest.mark.parametrize("B,nsig", [(16, 5), (64, 3), (128, 2)])
+def test_every_contraction_shape_gives_identical_results(ctx, torch, B, nsig):
+    # waves per workgroup x signals per wave only move work between waves and
+    # registers: every valid shape gives the same bits
+    N = 2 * B
+    s = torch.cuda.current_stream()
+    cs = np.concatenate([O.seeded_coeffs(B, 60 + i) for i in range(nsig)]).astype(np.complex64)
+    cdev = torch.from_numpy(cs).cuda()
+    outs = {}
+    for w in (1, 2, 4):
+        for sp in (1, 2, 4):
+            # (a shape a direction cannot take -- four signals per wave at 2B = 256,
+            # or for the inverse at 2B = 128 -- falls back to the default there)
+            ctx.tune("batch_force_shape", 10 * w + sp)
+            g = torch.empty(nsig * N ** 3, dtype=torch.complex64, device="cuda")
+            ctx.idsoft_batch_f32(B, nsig, cdev, g, stream=s)
+            c = torch.empty_like(cdev)
+            ctx.dsoft_batch_f32(B, nsig, g, c, stream=s)
+            outs[(w, sp)] = (g, c)
+    ctx.tune("batch_force_shape", 0)
+    torch.cuda.synchronize()
+    ref = outs[(4, 2)]
+    for k, (g, c) in outs.items():
+        assert torch.equal(g, ref[0]) and torch.equal(c, ref[1]), k
+
+
+def test_autotuner_picks_valid_shapes(ctx, torch):
+    ctx.autotune_batch(32, 16)
+    for fwd in (True, False):
+        for n in (1, 2, 3, 4, 8, 16, 40):
+            w, sp = ctx.batch_shape(32, fwd, n)
+            assert w in (1, 2, 4) and sp in (1, 2, 4)

@@ -148,3 +148,41 @@ def test_rejects_unserved_bandwidth_and_bad_kind(ctx, torch):
     srv.close()
     with pytest.raises(P.PfsError):
         P.Server(ctx, [12], slots=1, max_batch=2)
+
+
+def test_graph_replays_match_eager_batches(ctx, torch):
+    # batches replayed from captured graphs, with other work and synchronisations
+    # between the rounds, equal the eager batched entry points bit for bit
+    import pfsoftlib_amd as P
+    assert ctx is not None
+    for B in (16, 32):
+        N, n = 2 * B, O.coeff_count(B)
+        k = 3
+        c = torch.from_numpy(np.concatenate([O.seeded_coeffs(B, 400 + i) for i in range(k)]).astype(np.complex64)).cuda()
+        g_eager = torch.empty(k * N ** 3, dtype=torch.complex64, device="cuda")
+        c_eager = torch.empty_like(c)
+        ctx.idsoft_batch_f32(B, k, c, g_eager)
+        ctx.dsoft_batch_f32(B, k, g_eager, c_eager)
+        torch.cuda.synchronize()
+        for graphs in (1, 0):
+            ctx.tune("server_graphs", graphs)
+            srv = P.Server(ctx, [B], slots=1, max_batch=k)
+            for rnd in range(3):
+                gs = [torch.full((N ** 3,), float("nan"), dtype=torch.complex64, device="cuda") for _ in range(k)]
+                cs = [torch.full((n,), float("nan"), dtype=torch.complex64, device="cuda") for _ in range(k)]
+                inv = [srv.submit(P.PFS_REQ_INVERSE, B, c[i * n:(i + 1) * n], gs[i]) for i in range(k)]
+                for t in inv:
+                    srv.wait(t, timeout=60.0)
+                fwd = [srv.submit(P.PFS_REQ_FORWARD, B, g_eager[i * N ** 3:(i + 1) * N ** 3], cs[i]) for i in range(k)]
+                for t in fwd:
+                    srv.wait(t, timeout=60.0)
+                for i in range(k):
+                    assert torch.equal(gs[i], g_eager[i * N ** 3:(i + 1) * N ** 3]), (B, graphs, rnd, i)
+                    assert torch.equal(cs[i], c_eager[i * n:(i + 1) * n]), (B, graphs, rnd, i)
+                # eager work of the program between replays
+                ctx.idsoft_batch_f32(B, 1, c[:n], g_eager[:N ** 3])
+                torch.cuda.synchronize()
+            st = srv.stats()
+            assert (st["graphs"] > 0) == (graphs == 1)
+            srv.close()
+        ctx.tune("server_graphs", 1)
