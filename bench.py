@@ -345,6 +345,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--bandwidth", type=int, default=128)
     ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--power", type=float, default=0.0,
+                    help="coefficient decay exponent s(l) = (l+1)^-power (0 = 1.5 at B >= 256, else 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -354,6 +356,8 @@ def main():
                     help="signals per batched mixed-precision round trip at the same bandwidth (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-check", type=int, default=0, help="signals checked against FP64 (0 = all)")
+    ap.add_argument("--batch-fft-signals", type=int, default=-1, help="batched FFT chunk: signals (-1 = library default)")
+    ap.add_argument("--batch-fft-slices", type=int, default=-1, help="batched FFT chunk: beta-slices (-1 = library default)")
     ap.add_argument("--no-c4", action="store_true", help="skip the 512-signal B=64 batched leg")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="run the Poisson request stream through the request server for this long (0 = skip)")
@@ -383,13 +387,20 @@ def main():
     ctx.warm(B)
     ctx.tune("fft_chunk_slices", args.fft_chunk)
     ctx.tune("dwt_variant", args.dwt_variant)
-    coeffs = torch.from_numpy(P.seeded_coeffs(B, args.seed)).cuda()
+    if args.batch_fft_signals >= 0:
+        ctx.tune("batch_fft_signals", args.batch_fft_signals)
+    if args.batch_fft_slices >= 0:
+        ctx.tune("batch_fft_slices", args.batch_fft_slices)
+    power = args.power if args.power > 0 else (1.5 if B >= 256 else 1.0)
+    coeffs = torch.from_numpy(P.seeded_coeffs(B, args.seed, power)).cuda()
     grid = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
     back = torch.empty_like(coeffs)
     stream = torch.cuda.current_stream()
 
-    def step():
+    def step(e=None):
         ctx.idsoft_dev(B, coeffs, grid, stream=stream)
+        if e is not None:
+            e.record(stream)
         ctx.dsoft_dev(B, grid, back, stream=stream)
 
     for _ in range(args.warmup):
@@ -397,13 +408,13 @@ def main():
     torch.cuda.synchronize()
 
     ctx.timing(True)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         ev[i][0].record(stream)
-        step()
-        ev[i][1].record(stream)
+        step(ev[i][1])
+        ev[i][2].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.timing(False)
@@ -419,7 +430,13 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
     dk = kernels[dom]
 
-    lat = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps))
+    lat = sorted(ev[i][0].elapsed_time(ev[i][2]) for i in range(args.steps))
+    t_inv = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps))
+    t_fwd = sorted(ev[i][1].elapsed_time(ev[i][2]) for i in range(args.steps))
+
+    def pct(v):
+        return {"avg": sum(v) / len(v), "p50": v[len(v) // 2], "p99": v[min(len(v) - 1, int(0.99 * len(v)))]}
+
     err = (torch.linalg.vector_norm(back - coeffs) / torch.linalg.vector_norm(coeffs)).item()
     elapsed = t1 - t0
     out = {
@@ -434,7 +451,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: SplitMix64-seeded coefficients, uniform[-1,1]/(l+1)",
+        "data": "synthetic: SplitMix64-seeded coefficients, uniform[-1,1]/(l+1)^%g" % power,
         "config": {"workload": "plain FP64 single-signal DSOFT+iDSOFT round trip, B=%d, %d^3 complex128 grid, "
                                "%d coefficients, inputs resident in HBM" % (B, N, P.coeff_count(B)),
                    "bandwidth": B, "grid": [N, N, N], "precision": "fp64", "signals_per_step": 1},
@@ -442,6 +459,8 @@ def main():
                      "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": measured_traffic(B, dom),
                      "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"])},
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
+        "idsoft_ms": pct(t_inv),
+        "dsoft_ms": pct(t_fwd),
         "transforms_per_s": 2 * args.steps / elapsed,
         "roundtrip_rel_l2": err,
         "kernels": kernels,
@@ -455,7 +474,7 @@ def main():
         ctx.dsoft(B, g_host)
     out["host_buffer_pairs_per_s"] = 3 / (time.perf_counter() - t2)
     del grid, back
-    if args.batch_signals:
+    if args.batch_signals and B <= 128:  # the batched path serves B <= 128
         bl = batched_leg(torch, P, ctx, B, args.batch_signals, args.batch_steps, 1, args.seed, args.batch_check)
         out["batched_mixed"] = bl
         out["batched_vs_plain_fp64"] = bl["pairs_per_s"] / out["value"]

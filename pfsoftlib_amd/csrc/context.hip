@@ -263,6 +263,8 @@ struct pfs_context {
     long server_graphs = 1;     // request servers replay captured HIP graphs
     long server_autotune = 1;   // request servers autotune the batched shapes at creation
     long batch_force_shape = 0; // 0: tuned / default shapes; else 10 * waves + signals per wave
+    long batch_fft_signals = 0; // batched FFT: signals per chunk (0 = the whole sub-batch)
+    long batch_fft_slices = 0;  // batched FFT: beta-slices per chunk (0 = all); a multiple of 16
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -735,6 +737,17 @@ int batch_autotune(pfs_context* c, int B, int max_n)
     return PFS_OK;
 }
 
+int batch_fft_signals(const pfs_context* c, int n)
+{
+    return c->batch_fft_signals > 0 ? (int)std::min<long>(c->batch_fft_signals, n) : n;
+}
+
+int batch_fft_slices(const pfs_context* c, int N)
+{
+    const long k = c->batch_fft_slices;
+    return (k >= 16 && k < N && N % k == 0) ? (int)k : N;
+}
+
 // Forward batch of n <= cap signals: FFT rows (grids -> W), FFT columns
 // (W -> G), DWT (G -> coefficients).  W and G hold cap grids each.
 int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSet coeffs, float2* W, float2* G,
@@ -742,13 +755,29 @@ int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSe
 {
     const int B = p->B, N = p->t.N;
     const long N3 = (long)N * N * N, count = coeff_offset(B);
-    LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
-    a.in_sig = grids.stride;
-    a.in_ptrs = (const void* const*)grids.ptrs;
-    a.out_sig = b.in_sig = b.out_sig = N3;
     const float norm = (float)fwd_norm(B);
-    HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(grids.base, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
-    HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, G, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, n, st); }));
+    // slice FFTs in chunks of S signals x K slices: the row pass of a chunk
+    // writes a compact W block that the column pass reads back at once, so a
+    // chunk small enough stays in the Infinity Cache instead of going through HBM
+    const int S = batch_fft_signals(c, n), K = batch_fft_slices(c, N);
+    const long n2 = (long)N * N;
+    for (int g0 = 0; g0 < n; g0 += S) {
+        const int ns = std::min(S, n - g0);
+        const SigSet gs = grids.at(g0);
+        for (int k0 = 0; k0 < N; k0 += K) {
+            LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
+            a.nu = K;
+            a.in_sig = gs.stride;
+            a.in_ptrs = (const void* const*)gs.ptrs;
+            a.in_off = k0 * n2;
+            a.out_sig = b.in_sig = K * n2;
+            b.nv = K;
+            b.out_sig = N3;
+            float2* Gc = G + g0 * N3 + k0;
+            HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(gs.base, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, ns, st); }));
+            HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, Gc, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, ns, st); }));
+        }
+    }
     if (c->batch_interleave || coeffs.ptrs) {
         // W is free once the column pass has run: the coefficients go there interleaved
         const int ld = interleave_ld(n, cap, count, N3);
@@ -767,10 +796,6 @@ int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, Sig
 {
     const int B = p->B, N = p->t.N;
     const long N3 = (long)N * N * N, count = coeff_offset(B);
-    LineMap b = map_cols_inv(N), a = map_rows_inv(N);
-    a.out_sig = grids.stride;
-    a.out_ptrs = (void* const*)grids.ptrs;
-    a.in_sig = b.in_sig = b.out_sig = N3;
     const float norm = (float)inv_norm(B);
     const float scale = (float)(1. / (4. * B * B));
     HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(G, B, n, st); }));
@@ -781,8 +806,25 @@ int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, Sig
     } else {
         HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, shape_for(c, *p, false, n), st); }));
     }
-    HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(G, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
-    HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, grids.base, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, n, st); }));
+    const int S = batch_fft_signals(c, n), K = batch_fft_slices(c, N);
+    const long n2 = (long)N * N;
+    for (int g0 = 0; g0 < n; g0 += S) {
+        const int ns = std::min(S, n - g0);
+        const SigSet gs = grids.at(g0);
+        for (int k0 = 0; k0 < N; k0 += K) {
+            LineMap b = map_cols_inv(N), a = map_rows_inv(N);
+            b.nv = K;
+            b.in_sig = N3;
+            b.out_sig = a.in_sig = K * n2;
+            a.nu = K;
+            a.out_sig = gs.stride;
+            a.out_ptrs = (void* const*)gs.ptrs;
+            a.out_off = k0 * n2;
+            const float2* Gc = G + g0 * N3 + k0;
+            HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(Gc, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, ns, st); }));
+            HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, gs.base, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, ns, st); }));
+        }
+    }
     return PFS_OK;
 }
 
@@ -1145,6 +1187,16 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->server_autotune = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "batch_fft_signals")) {
+        if (value < 0) return fail(PFS_ERR_ARG, "batch_fft_signals must be >= 0");
+        c->batch_fft_signals = value;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_fft_slices")) {
+        if (value < 0 || (value && (value % 16))) return fail(PFS_ERR_ARG, "batch_fft_slices must be 0 or a multiple of 16");
+        c->batch_fft_slices = value;
+        return PFS_OK;
+    }
     if (!strcmp(key, "batch_force_shape")) {
         const BatchKernelShape sh{(int)value / 10, (int)value % 10};
         if (value != 0 && !(batch_shape_valid(256, true, sh) || batch_shape_valid(64, true, sh)))
@@ -1181,6 +1233,14 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "server_autotune")) {
         *value = c->server_autotune;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_fft_signals")) {
+        *value = c->batch_fft_signals;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_fft_slices")) {
+        *value = c->batch_fft_slices;
         return PFS_OK;
     }
     if (!strcmp(key, "batch_force_shape")) {

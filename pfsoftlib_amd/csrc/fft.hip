@@ -197,12 +197,12 @@ __device__ __forceinline__ void store_tile(const C* lds, int ls, C* __restrict__
 template <typename C>
 __device__ __forceinline__ const C* sig_in(const C* in, const LineMap& m, long sig)
 {
-    return m.in_ptrs ? reinterpret_cast<const C*>(m.in_ptrs[sig]) : in + sig * m.in_sig;
+    return (m.in_ptrs ? reinterpret_cast<const C*>(m.in_ptrs[sig]) : in + sig * m.in_sig) + m.in_off;
 }
 template <typename C>
 __device__ __forceinline__ C* sig_out(C* out, const LineMap& m, long sig)
 {
-    return m.out_ptrs ? reinterpret_cast<C*>(m.out_ptrs[sig]) : out + sig * m.out_sig;
+    return (m.out_ptrs ? reinterpret_cast<C*>(m.out_ptrs[sig]) : out + sig * m.out_sig) + m.out_off;
 }
 
 template <typename T, int LOGN, int LINES, int KIND, int SIGN>

@@ -85,6 +85,7 @@ struct LineMap {
     // instead of in + s * in_sig; device arrays of device addresses
     const void* const* in_ptrs;
     void* const* out_ptrs;
+    long in_off, out_off;  // complex elements added to every signal's start
 };
 
 enum LineKind { LINES_ROW_IN_TILE_OUT = 0, LINES_TILE_IN_ROW_OUT = 1 };

@@ -137,3 +137,26 @@ def test_autotuner_picks_valid_shapes(ctx, torch):
         for n in (1, 2, 3, 4, 8, 16, 40):
             w, sp = ctx.batch_shape(32, fwd, n)
             assert w in (1, 2, 4) and sp in (1, 2, 4)
+
+
+@pytest.mark.parametrize("B,nsig", [(64, 5), (128, 3)])
+def test_chunked_batch_fft_is_bitwise_invariant(ctx, torch, B, nsig):
+    # the slice FFTs in chunks of signals x slices do the same arithmetic per line
+    N = 2 * B
+    s = torch.cuda.current_stream()
+    cs = np.concatenate([O.seeded_coeffs(B, 80 + i) for i in range(nsig)]).astype(np.complex64)
+    cdev = torch.from_numpy(cs).cuda()
+    outs = []
+    for S, K in [(0, 0), (2, 16), (3, 32), (1, 0), (0, 64)]:
+        ctx.tune("batch_fft_signals", S)
+        ctx.tune("batch_fft_slices", K)
+        g = torch.empty(nsig * N ** 3, dtype=torch.complex64, device="cuda")
+        ctx.idsoft_batch_f32(B, nsig, cdev, g, stream=s)
+        c = torch.empty_like(cdev)
+        ctx.dsoft_batch_f32(B, nsig, g, c, stream=s)
+        outs.append((g, c))
+    ctx.tune("batch_fft_signals", 0)
+    ctx.tune("batch_fft_slices", 0)
+    torch.cuda.synchronize()
+    for g, c in outs[1:]:
+        assert torch.equal(g, outs[0][0]) and torch.equal(c, outs[0][1])
