@@ -285,30 +285,34 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     const int j = lane & 15, bsel = lane >> 4;
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
     const double* csrc = reinterpret_cast<const double*>(coeffs) + (j & 1);
-    auto load_b = [&](int chk, double* b) {
+    // Gathers of chunk chk: raw values and their factor n' * sign (0 past R and
+    // for inactive cases, where coefficient 0 stands in); unconditional loads,
+    // scaled at use, so they stay in flight across the chunk's work.
+    auto load_b = [&](int chk, double* raw, double* fac) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int r = chk * 16 + 4 * s + bsel;
-            double v = 0.0;
-            if (lc.active && r < R) {
-                const double sg = (r & 1) ? lc.sgn_odd : lc.sgn;
-                v = sg * (norm * csrc[2 * coeff_index32(M + r, lc.Mc, lc.Mpc)]);
-            }
-            b[s] = v;
+            const bool live = lc.active && r < R;
+            const int ci = live ? 2 * coeff_index32(M + r, lc.Mc, lc.Mpc) : 0;
+            fac[s] = live ? norm * ((r & 1) ? lc.sgn_odd : lc.sgn) : 0.0;
+            raw[s] = csrc[ci];
         }
     };
 
     v4d acc[NCH * 4];
 #pragma unroll
     for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
-    double bcur[4], bnext[4];
-    load_b(0, bcur);
+    double rcur[4], fcur[4], rnext[4], fnext[4];
+    load_b(0, rcur, fcur);
 
     for (int chk = 0; chk < nch; ++chk) {
-        if (chk + 1 < nch) load_b(chk + 1, bnext);
+        load_b(chk + 1 < nch ? chk + 1 : chk, rnext, fnext);
         __syncthreads();
         run_chains<NCH, 0, NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
         __syncthreads();
+        double bcur[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bcur[s] = rcur[s] * fcur[s];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -320,7 +324,10 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
                         acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[s], acc[4 * c + kb], 0, 0, 0);
                     }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) bcur[s] = bnext[s];
+        for (int s = 0; s < 4; ++s) {
+            rcur[s] = rnext[s];
+            fcur[s] = fnext[s];
+        }
     }
 
     // stage each chain's 64 x 16 block and write each case's beta-vector run
