@@ -160,7 +160,7 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *                       (0 default, 1..3 alternatives; results equal).
  *   "server_graphs"     see the request server below.
  *   "fft_chunk_slices"  beta-slices per FFT chunk (0 = automatic: one chunk);
- *                       rounded to a power of two dividing 2B.  Smaller
+ *                       rounded to a power of two >= 16 dividing 2B.  Smaller
  *                       chunks keep the row-pass output in the Infinity
  *                       Cache at the price of more, smaller launches.
  *   "batch_chunk_signals" signals per sub-batch of the batched path (work

@@ -485,7 +485,7 @@ int fft_chunk(const pfs_context* c, int N)
 {
     long want = c->fft_chunk_slices;
     if (want <= 0) return N;
-    int k = 8;
+    int k = 16;  // the column pass moves 16 slices per workgroup
     while (k * 2 <= want && k * 2 <= N) k *= 2;
     if (k > N || N % k) k = N;
     return k;

@@ -331,12 +331,21 @@ template <int Rn, int SIGN, typename C> __device__ __forceinline__ void dft_smal
     else dft2<SIGN>(v);
 }
 
-// Lines per workgroup of the register kernel: 128 bytes of one column element
-// per line group, so the tile-side accesses are whole 128-byte runs.
-template <typename T> constexpr int r16_lines() { return 128 / (2 * (int)sizeof(T)); }
+// Lines per workgroup of the register kernel: the tile-side accesses move
+// whole runs of one column element over the workgroup's lines.
+template <typename T, int KIND> constexpr int r16_lines()
+{
+    // measured on MI355X at 2B = 256: complex128 tile writes prefer 8 lines
+    return (sizeof(T) == 8 && KIND == LINES_ROW_IN_TILE_OUT) ? 8 : 16;
+}
+// LDS bytes of the register kernel: L lines of 16 rows x (TPL + 1) + 1 complex.
+template <typename T, int KIND> constexpr size_t r16_lds(int logn)
+{
+    return (size_t)r16_lines<T, KIND>() * (16 * ((1 << logn) / 16 + 1) + 1) * 2 * sizeof(T);
+}
 
 template <typename T, int LOGN, int KIND, int SIGN>
-__global__ __launch_bounds__(r16_lines<T>() * (1 << LOGN) / 16) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
+__global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
                                                                           typename CT<T>::C* __restrict__ out,
                                                                           LineMap m,
                                                                           const typename CT<T>::C* __restrict__ tw,
@@ -345,10 +354,11 @@ __global__ __launch_bounds__(r16_lines<T>() * (1 << LOGN) / 16) void k_fft_lines
     typedef typename CT<T>::C C;
     constexpr int N = 1 << LOGN;
     constexpr int TPL = N / 16;          // threads per line; also the step-B DFT size
-    constexpr int L = r16_lines<T>();    // lines per workgroup
+    constexpr int L = r16_lines<T, KIND>();  // lines per workgroup
     constexpr int YS = TPL + 1;          // LDS row stride (complex) for one k2
     constexpr int LS = 16 * YS + 1;      // LDS stride between lines
-    __shared__ C lds[L * LS];
+    extern __shared__ __align__(16) unsigned char smem_r16[];
+    C* lds = reinterpret_cast<C*>(smem_r16);  // L * LS complex
 
     const int tid = threadIdx.x;
     const long u = blockIdx.y, v0 = (long)blockIdx.x * L, sig = blockIdx.z;
@@ -407,11 +417,13 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
 {
     constexpr int N = 1 << LOGN;
     if (LOGN >= 6 && LOGN <= 8) {
-        constexpr int L = r16_lines<T>();
+        constexpr int L = r16_lines<T, KIND>();
         if (m.nv % L != 0) return hipErrorInvalidValue;
         dim3 grid(m.nv / L, m.nu, nsig);
+        const size_t smem = r16_lds<T, KIND>(LOGN);
+        if (smem > lds_limit()) return hipErrorInvalidConfiguration;
         hipLaunchKernelGGL((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
-                           dim3(L * N / 16), 0, st, in, out, m, tw, scale);
+                           dim3(L * N / 16), smem, st, in, out, m, tw, scale);
         return hipGetLastError();
     }
     constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;
