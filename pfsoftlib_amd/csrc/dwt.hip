@@ -150,6 +150,14 @@ __device__ __forceinline__ void init_chains(Chains<NCH>& ch, const ChainTables<N
     }
 }
 
+// Orders one wave's LDS accesses around a point: its earlier LDS operations
+// complete before any later one (no other wave is waited for).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Broadcast lane `src` (uniform) of a double to every lane, via SGPRs.
 __device__ __forceinline__ double bcast_lane(double v, int src)
 {
@@ -367,12 +375,14 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     v4d acc[NCH * 4];
 #pragma unroll
     for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
+    __syncthreads();  // the step-coefficient table is complete; from here on a
+                      // wave touches only its own tile, so it syncs with itself
 
     for (int chk = 0; chk < nch; ++chk) {
         load_b(chk + 1 < nch ? chk + 1 : chk, rnext, fnext);
-        __syncthreads();
+        wave_sync();
         run_chains<NCH, 0, NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
-        __syncthreads();
+        wave_sync();
         double bcur[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) bcur[s] = rcur[s] * fcur[s];
@@ -406,14 +416,14 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     double* stage = tile;  // 64 rows x OSTR fit in the wave's region
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
             if (!SMALL || kb < nkb)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                     stage[(16 * kb + mfma_out_row(lane, v)) * OSTR + mfma_out_col(lane)] = acc[4 * c + kb][v];
-        __syncthreads();
+        wave_sync();
         const int k = kw + 64 * c + lane;
         if (k < N) {
 #pragma unroll
