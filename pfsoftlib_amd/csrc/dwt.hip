@@ -150,6 +150,19 @@ __device__ __forceinline__ void init_chains(Chains<NCH>& ch, const ChainTables<N
     }
 }
 
+// Pair handled by workgroup b.  Workgroups are dealt to the 8 XCDs round-robin;
+// within each window of 64 pairs (heaviest first overall) this gives every XCD
+// a run of 8 consecutive pairs -- same M, consecutive M' -- whose coefficients
+// (l, M', M) etc. sit next to each other, so one L2 serves the neighbouring
+// reads and merges the neighbouring partial-line writes.  The last, partial
+// window keeps the identity order.
+__device__ __forceinline__ int xcd_pair(int b, int npairs)
+{
+    const int g = b >> 6, r = b & 63;
+    if ((g + 1) * 64 > npairs) return b;
+    return g * 64 + (r & 7) * 8 + (r >> 3);
+}
+
 // Orders one wave's LDS accesses around a point: its earlier LDS operations
 // complete before any later one (no other wave is waited for).
 __device__ __forceinline__ void wave_sync()
@@ -232,7 +245,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NH);
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[blockIdx.x];
+    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;
@@ -331,7 +344,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     constexpr int TS = tile_stride(NCH);
     constexpr int WREG = inv_wave_region(NCH);  // a wave's tile, reused as its output staging
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[blockIdx.x];
+    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;
@@ -386,16 +399,25 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
         double bcur[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) bcur[s] = rcur[s] * fcur[s];
+        // all of the chunk's A operands first, then the MFMAs: left alone the
+        // compiler reads one operand pair per MFMA pair and waits on each read
+        double a[4][NCH][4];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb)
-                    if (!SMALL || kb < nkb) {
-                        double a = tile[(4 * s + bsel) * TS + 64 * c + 16 * kb + j];
-                        acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[s], acc[4 * c + kb], 0, 0, 0);
-                    }
+                    a[s][c][kb] = (!SMALL || kb < nkb) ? tile[(4 * s + bsel) * TS + 64 * c + 16 * kb + j] : 0.0;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+                    if (!SMALL || kb < nkb)
+                        acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][c][kb], bcur[s], acc[4 * c + kb], 0, 0, 0);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             rcur[s] = rnext[s];
@@ -461,7 +483,6 @@ static DwtShape dwt_shape(int N, bool fwd, int variant)
     }
     switch (variant) {
     case 1: return {1, 1, 4};
-    case 2: return {4, 4, 1};
     default: return {2, 2, 2};
     }
 }
@@ -518,7 +539,6 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     PFS_INV(2, 2)
     PFS_INV(1, 2)
     PFS_INV(1, 4)
-    PFS_INV(4, 1)
     else return hipErrorInvalidValue;
 #undef PFS_INV
     return hipGetLastError();
