@@ -17,6 +17,22 @@ namespace lg {
 
 constexpr int OSTR = 18;            // output staging row stride (doubles), 64 rows
 
+// Orders one wave's LDS accesses around a point (no other wave is waited for).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Pair handled by workgroup b: runs of 8 consecutive pairs per XCD within
+// windows of 64 (round-robin XCD dispatch), identity in the last partial window.
+__device__ __forceinline__ int xcd_pair(int b, int npairs)
+{
+    const int g = b >> 6, r = b & 63;
+    if ((g + 1) * 64 > npairs) return b;
+    return g * 64 + (r & 7) * 8 + (r >> 3);
+}
+
 struct CaseInfo {
     long vec;     // pencil offset (complex elements) of the (p, q) beta-vector
     int Mc, Mpc;  // coefficient orders
@@ -172,7 +188,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NH);
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[blockIdx.x];
+    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;
@@ -264,7 +280,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NCH);
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[blockIdx.x];
+    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;
@@ -304,25 +320,35 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     for (int i = 0; i < NCH * 4; ++i) acc[i] = (v4d){0.0, 0.0, 0.0, 0.0};
     double rcur[4], fcur[4], rnext[4], fnext[4];
     load_b(0, rcur, fcur);
+    __syncthreads();  // the step-coefficient table is complete; from here on a
+                      // wave touches only its own tile, so it syncs with itself
 
     for (int chk = 0; chk < nch; ++chk) {
         load_b(chk + 1 < nch ? chk + 1 : chk, rnext, fnext);
-        __syncthreads();
+        wave_sync();
         run_chains<NCH, 0, NCH>(ch, tile, TS, lane, cf, RP, chk * 16);
-        __syncthreads();
+        wave_sync();
         double bcur[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) bcur[s] = rcur[s] * fcur[s];
+        // all of the chunk's A operands first, then the MFMAs
+        double a[4][NCH][4];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb)
-                    if (!SMALL || kb < nkb) {
-                        double a = tile[(4 * s + bsel) * TS + 64 * c + 16 * kb + j];
-                        acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[s], acc[4 * c + kb], 0, 0, 0);
-                    }
+                    a[s][c][kb] = (!SMALL || kb < nkb) ? tile[(4 * s + bsel) * TS + 64 * c + 16 * kb + j] : 0.0;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+                    if (!SMALL || kb < nkb)
+                        acc[4 * c + kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][c][kb], bcur[s], acc[4 * c + kb], 0, 0, 0);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             rcur[s] = rnext[s];
@@ -331,17 +357,17 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     }
 
     // stage each chain's 64 x 16 block and write each case's beta-vector run
-    double* stage = tile;  // 64 * OSTR <= 16 * TS
+    double* stage = tile;  // 64 * OSTR <= 16 * TS for the two-chain shapes used here
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
             if (!SMALL || kb < nkb)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                     stage[(16 * kb + mfma_out_row(lane, v)) * OSTR + mfma_out_col(lane)] = acc[4 * c + kb][v];
-        __syncthreads();
+        wave_sync();
         const int k = kw + 64 * c + lane;
         if (k < N) {
             for (int cs = 0; cs < 8; ++cs) {
