@@ -205,6 +205,30 @@ __device__ __forceinline__ C* sig_out(C* out, const LineMap& m, long sig)
     return (m.out_ptrs ? reinterpret_cast<C*>(m.out_ptrs[sig]) : out + sig * m.out_sig) + m.out_off;
 }
 
+// Grid data lives in device global memory; saying so lets the compiler use
+// global (not flat) loads and stores, which count only against vmcnt.  The
+// accesses go through clang vector types (the HIP double2 / float2 classes
+// have no assignment from another address space).
+template <typename C> struct GV;
+template <> struct GV<double2> { typedef double T __attribute__((ext_vector_type(2))); };
+template <> struct GV<float2> { typedef float T __attribute__((ext_vector_type(2))); };
+template <typename C> using gvec = __attribute__((address_space(1))) typename GV<C>::T;
+template <typename C> __device__ __forceinline__ C gload(const C* base, unsigned i)
+{
+    const typename GV<C>::T v = ((const gvec<C>*)base)[i];
+    C r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+template <typename C> __device__ __forceinline__ void gstore(C* base, unsigned i, C r)
+{
+    typename GV<C>::T v;
+    v.x = r.x;
+    v.y = r.y;
+    ((gvec<C>*)base)[i] = v;
+}
+
 template <typename T, int LOGN, int LINES, int KIND, int SIGN>
 __global__ __launch_bounds__(256) void k_fft_lines(const typename CT<T>::C* __restrict__ in,
                                                    typename CT<T>::C* __restrict__ out, LineMap m,
@@ -338,10 +362,24 @@ template <typename T, int KIND> constexpr int r16_lines()
     // measured on MI355X at 2B = 256: complex128 tile writes prefer 8 lines
     return (sizeof(T) == 8 && KIND == LINES_ROW_IN_TILE_OUT) ? 8 : 16;
 }
-// LDS bytes of the register kernel: L lines of 16 rows x (TPL + 1) + 1 complex.
+// y * w for a twiddle w = tw[e] of the forward table exp(-2 pi i e / N),
+// conjugated for SIGN > 0.  tw[0] = 1 exactly, so e = 0 needs no branch (a
+// branch around the load would make every load wait for the previous one).
+template <int SIGN, typename C> __device__ __forceinline__ C twiddle(C y, C w)
+{
+    if (SIGN > 0) w.y = -w.y;
+    return cmul(y, w);
+}
+
+// The register kernel exchanges the real parts through LDS, then the
+// imaginary parts, in the same buffer: half the LDS of a complex exchange, so
+// twice the workgroups fit on a CU (its registers allow four waves per SIMD)
+// for two more barriers.
+template <typename T> constexpr bool r16_split() { return sizeof(T) == 8; }
+// LDS bytes of the register kernel: L lines of 16 rows x (TPL + 1) + 1 values.
 template <typename T, int KIND> constexpr size_t r16_lds(int logn)
 {
-    return (size_t)r16_lines<T, KIND>() * (16 * ((1 << logn) / 16 + 1) + 1) * 2 * sizeof(T);
+    return (size_t)r16_lines<T, KIND>() * (16 * ((1 << logn) / 16 + 1) + 1) * (r16_split<T>() ? 1 : 2) * sizeof(T);
 }
 
 template <typename T, int LOGN, int KIND, int SIGN>
@@ -355,10 +393,9 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
     constexpr int N = 1 << LOGN;
     constexpr int TPL = N / 16;          // threads per line; also the step-B DFT size
     constexpr int L = r16_lines<T, KIND>();  // lines per workgroup
-    constexpr int YS = TPL + 1;          // LDS row stride (complex) for one k2
+    constexpr int YS = TPL + 1;          // LDS row stride (values) for one k2
     constexpr int LS = 16 * YS + 1;      // LDS stride between lines
     extern __shared__ __align__(16) unsigned char smem_r16[];
-    C* lds = reinterpret_cast<C*>(smem_r16);  // L * LS complex
 
     const int tid = threadIdx.x;
     const long u = blockIdx.y, v0 = (long)blockIdx.x * L, sig = blockIdx.z;
@@ -370,43 +407,71 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
     if (KIND == LINES_ROW_IN_TILE_OUT) { n1 = tid % TPL; la = tid / TPL; }
     else { la = tid % L; n1 = tid / L; }
     C v[16];
+    {
+        // offsets within one signal's grid stay below N^3 <= 2^27: 32-bit
+        const unsigned o = (unsigned)(la * m.in_sv + n1 * m.in_sn), step = (unsigned)(TPL * m.in_sn);
 #pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[la * m.in_sv + (long)(n1 + TPL * n2) * m.in_sn];
-    dft16_perm<SIGN>(v);
-    C* ly = lds + la * LS;
-#pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) {
-        C y = v[perm_index<16>(k2)];
-        const int e = n1 * k2;  // < N
-        if (e) {
-            C w = tw[e];
-            if (SIGN > 0) w.y = -w.y;
-            y = cmul(y, w);
-        }
-        ly[k2 * YS + n1] = y;
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = gload(src, o + n2 * step);
     }
-    __syncthreads();
+    // the twiddles of this thread's 16 outputs, in flight with the line loads
+    C w[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) w[k2] = gload(tw, n1 * k2);
+    dft16_perm<SIGN>(v);
 
     // step B: 16 / TPL DFTs of size TPL per thread; row stores with k2 fastest,
     // tile stores with the line fastest
     int lb, t;
     if (KIND == LINES_ROW_IN_TILE_OUT) { lb = tid % L; t = tid / L; }
     else { t = tid % TPL; lb = tid / TPL; }
-    const C* lyb = lds + lb * LS;
     constexpr int Q = 16 / TPL;
+    C x[Q][TPL];
+    if (r16_split<T>()) {
+        // real parts go to LDS as they are twiddled; imaginary parts wait in registers
+        T* ly = reinterpret_cast<T*>(smem_r16) + la * LS;
+        const T* lyb = reinterpret_cast<const T*>(smem_r16) + lb * LS;
+        T yi[16];
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            C y = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+            ly[k2 * YS + n1] = y.x;
+            yi[k2] = y.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int a = 0; a < TPL; ++a) x[q][a].x = lyb[(t + TPL * q) * YS + a];
+        __syncthreads();
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) ly[k2 * YS + n1] = yi[k2];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int a = 0; a < TPL; ++a) x[q][a].y = lyb[(t + TPL * q) * YS + a];
+    } else {
+        C* ly = reinterpret_cast<C*>(smem_r16) + la * LS;
+        const C* lyb = reinterpret_cast<const C*>(smem_r16) + lb * LS;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) ly[k2 * YS + n1] = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int a = 0; a < TPL; ++a) x[q][a] = lyb[(t + TPL * q) * YS + a];
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int k2 = t + TPL * q;
-        C x[TPL];
-#pragma unroll
-        for (int a = 0; a < TPL; ++a) x[a] = lyb[k2 * YS + a];
-        dft_small<TPL, SIGN>(x);
+        dft_small<TPL, SIGN>(x[q]);
+        const unsigned o = (unsigned)(lb * m.out_sv + k2 * m.out_sn), step = (unsigned)(16 * m.out_sn);
 #pragma unroll
         for (int k1 = 0; k1 < TPL; ++k1) {
-            C r = x[perm_index<TPL>(k1)];
+            C r = x[q][perm_index<TPL>(k1)];
             r.x *= scale;
             r.y *= scale;
-            dst[lb * m.out_sv + (long)(k2 + 16 * k1) * m.out_sn] = r;
+            gstore(dst, o + k1 * step, r);
         }
     }
 }
