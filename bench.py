@@ -397,24 +397,25 @@ def main():
     back = torch.empty_like(coeffs)
     stream = torch.cuda.current_stream()
 
-    def step(e=None):
+    def step():
         ctx.idsoft_dev(B, coeffs, grid, stream=stream)
-        if e is not None:
-            e.record(stream)
         ctx.dsoft_dev(B, grid, back, stream=stream)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
+    # per-kernel times: start/stop events in every launch's own dispatch packet
+    # (no extra packets in the stream); per-step latency: one event per step
+    # boundary (step i runs from marks[i] to marks[i + 1])
     ctx.timing(True)
-    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(args.steps)]
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    marks[0].record(stream)
     for i in range(args.steps):
-        ev[i][0].record(stream)
-        step(ev[i][1])
-        ev[i][2].record(stream)
+        step()
+        marks[i + 1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.timing(False)
@@ -430,12 +431,10 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
     dk = kernels[dom]
 
-    lat = sorted(ev[i][0].elapsed_time(ev[i][2]) for i in range(args.steps))
-    t_inv = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps))
-    t_fwd = sorted(ev[i][1].elapsed_time(ev[i][2]) for i in range(args.steps))
+    lat = sorted(marks[i].elapsed_time(marks[i + 1]) for i in range(args.steps))
 
-    def pct(v):
-        return {"avg": sum(v) / len(v), "p50": v[len(v) // 2], "p99": v[min(len(v) - 1, int(0.99 * len(v)))]}
+    def kernel_ms(names):
+        return round(sum(kernels[k]["us_per_step"] for k in names if k in kernels) / 1e3, 4)
 
     err = (torch.linalg.vector_norm(back - coeffs) / torch.linalg.vector_norm(coeffs)).item()
     elapsed = t1 - t0
@@ -459,8 +458,9 @@ def main():
                      "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": measured_traffic(B, dom),
                      "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"])},
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
-        "idsoft_ms": pct(t_inv),
-        "dsoft_ms": pct(t_fwd),
+        # kernel time per direction (sum of its kernels' average durations)
+        "idsoft_ms": {"kernels_avg": kernel_ms(["dwt_inv", "nyquist_gather", "fft_inv_cols", "fft_inv_rows"])},
+        "dsoft_ms": {"kernels_avg": kernel_ms(["fft_fwd_rows", "fft_fwd_cols", "dwt_fwd"])},
         "transforms_per_s": 2 * args.steps / elapsed,
         "roundtrip_rel_l2": err,
         "kernels": kernels,

@@ -160,7 +160,7 @@ int mfma_selftest_f32(hipStream_t st)
 {
     float* d = nullptr;
     HIP_TRY(hipMalloc(&d, 512 * sizeof(float)));
-    hipLaunchKernelGGL(k_mfma_probe_f32, dim3(1), dim3(64), 0, st, d);
+    pfs::launch(k_mfma_probe_f32, dim3(1), dim3(64), 0, st, d);
     float h[512];
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
@@ -192,7 +192,7 @@ int mfma_selftest(hipStream_t st)
 {
     double* d = nullptr;
     HIP_TRY(hipMalloc(&d, 512 * sizeof(double)));
-    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, d);
+    pfs::launch(k_mfma_probe, dim3(1), dim3(64), 0, st, d);
     double h[512];
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
@@ -270,6 +270,12 @@ struct pfs_context {
     size_t bwork_bytes = 0;
 };
 
+LaunchTiming& pfs::pending_timing()
+{
+    static thread_local LaunchTiming t = {nullptr, nullptr};
+    return t;
+}
+
 namespace {
 
 hipEvent_t take_event(pfs_context* c)
@@ -279,21 +285,32 @@ hipEvent_t take_event(pfs_context* c)
         c->ev_pool.pop_back();
         return e;
     }
+    // timing only: no system-scope fence (cache writeback + invalidate) at each
+    // record, which would otherwise slow the launch that follows
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
-// Wraps one kernel launch: records events around it when timing is enabled.
+// Wraps one kernel launch: when timing is enabled the launch records a start
+// and a stop event in its own dispatch packet.
 template <typename F>
 hipError_t timed(pfs_context* c, const char* name, hipStream_t st, F&& launch)
 {
     if (!c->timing) return launch();
     hipEvent_t a = take_event(c), b = take_event(c);
-    if (a) (void)hipEventRecord(a, st);
+    LaunchTiming& t = pending_timing();
+    t.start = a;
+    t.stop = b;
     hipError_t e = launch();
-    if (b) (void)hipEventRecord(b, st);
-    if (a && b) c->recs.push_back({name, a, b});
+    const bool taken = t.start == nullptr && t.stop == nullptr;
+    t.start = t.stop = nullptr;
+    if (a && b && taken && e == hipSuccess) {
+        c->recs.push_back({name, a, b});
+    } else {
+        if (a) c->ev_pool.push_back(a);
+        if (b) c->ev_pool.push_back(b);
+    }
     return e;
 }
 
@@ -377,10 +394,10 @@ int get_plan(pfs_context* c, int B, Plan** out)
     hipError_t e = hipMemcpyAsync((void*)p.t.pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
+        pfs::launch(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
                            (double*)p.t.sinh, (double*)p.t.cosh, (double*)p.t.wts, (double2*)p.t.tw,
                            (float2*)p.t.tw32);
-        hipLaunchKernelGGL(k_plan_normfac, dim3(B), dim3(64), 0, c->stream, B, (double*)p.t.normfac);
+        pfs::launch(k_plan_normfac, dim3(B), dim3(64), 0, c->stream, B, (double*)p.t.normfac);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1077,7 +1094,7 @@ int pfs_wigner_d_matrix(pfs_context* c, int B, int M, int Mp, int weighted, doub
     size_t bytes = (size_t)(B - M) * 2 * B * sizeof(double);
     rc = ensure_io(c, bytes, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_wigner_table, dim3((2 * B + 63) / 64), dim3(64), 0, c->stream, B, M, Mp, p->t.cosb,
+    pfs::launch(k_wigner_table, dim3((2 * B + 63) / 64), dim3(64), 0, c->stream, B, M, Mp, p->t.cosb,
                        p->t.sinh, p->t.cosh, p->t.wts, p->t.normfac, weighted, (double*)c->io_grid);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, c->io_grid, bytes, hipMemcpyDeviceToHost, c->stream));

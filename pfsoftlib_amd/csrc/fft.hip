@@ -487,7 +487,7 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
         dim3 grid(m.nv / L, m.nu, nsig);
         const size_t smem = r16_lds<T, KIND>(LOGN);
         if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
+        pfs::launch((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
                            dim3(L * N / 16), smem, st, in, out, m, tw, scale);
         return hipGetLastError();
     }
@@ -496,7 +496,7 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     if (m.nv % LINES != 0) return hipErrorInvalidValue;
     dim3 grid(m.nv / LINES, m.nu, nsig);
-    hipLaunchKernelGGL((k_fft_lines<T, LOGN, LINES, KIND, SIGN>), grid, dim3(256), smem, st, in, out, m, tw, scale);
+    pfs::launch((k_fft_lines<T, LOGN, LINES, KIND, SIGN>), grid, dim3(256), smem, st, in, out, m, tw, scale);
     return hipGetLastError();
 }
 
@@ -547,14 +547,14 @@ static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* o
     dim3 grid(m.nv / lines, m.nu, nsig);
     if (kind == LINES_ROW_IN_TILE_OUT) {
         if (sign < 0)
-            hipLaunchKernelGGL((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+            pfs::launch((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
         else
-            hipLaunchKernelGGL((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+            pfs::launch((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
     } else {
         if (sign < 0)
-            hipLaunchKernelGGL((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+            pfs::launch((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
         else
-            hipLaunchKernelGGL((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
+            pfs::launch((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
     }
     return hipGetLastError();
 }

@@ -12,10 +12,46 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <tuple>
+#include <utility>
 
 namespace pfs {
+
+// ---- kernel launches -----------------------------------------------------
+// Start/stop events the next kernel launch of this host thread records in its
+// own dispatch packet (hipExtLaunchKernel), set by the context's timing wrapper
+// around exactly one launch.  No barrier packets go into the stream, so timing
+// a sequence of kernels does not slow it down.
+struct LaunchTiming {
+    hipEvent_t start, stop;
+};
+LaunchTiming& pending_timing();
+
+template <typename... Formals, size_t... I>
+inline hipError_t launch_args(void (*kernel)(Formals...), dim3 grid, dim3 block, size_t smem, hipStream_t st,
+                              std::tuple<Formals...>& f, std::index_sequence<I...>)
+{
+    void* args[] = {(void*)&std::get<I>(f)..., nullptr};
+    LaunchTiming& t = pending_timing();
+    const hipEvent_t a = t.start, b = t.stop;
+    t.start = t.stop = nullptr;
+    if (a || b) return hipExtLaunchKernel((const void*)kernel, grid, block, args, smem, st, a, b, 0);
+    return hipLaunchKernel((const void*)kernel, grid, block, args, smem, st);
+}
+
+// Launches kernel<<<grid, block, smem, st>>>(actuals...), the actuals converted
+// to the kernel's parameter types.
+template <typename... Formals, typename... Actuals>
+inline hipError_t launch(void (*kernel)(Formals...), dim3 grid, dim3 block, size_t smem, hipStream_t st,
+                         Actuals&&... actuals)
+{
+    static_assert(sizeof...(Formals) == sizeof...(Actuals), "kernel argument count");
+    std::tuple<Formals...> f(std::forward<Actuals>(actuals)...);
+    return launch_args(kernel, grid, block, smem, st, f, std::index_sequence_for<Formals...>{});
+}
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
