@@ -207,6 +207,19 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__
 // 16 rows an MFMA operand read touches fall in distinct banks).
 __host__ __device__ constexpr int tile_stride(int nc) { return 64 * nc + 1; }
 
+// Pencil staging of the forward preamble: 8 cases x 64 nodes x (re, im) for one
+// chain, case rows padded to 136 doubles (272 words = 16 mod 64) so the
+// operand-layout reads spread over all banks.
+constexpr int FSTG_CASE = 136;
+constexpr int FSTG = 8 * FSTG_CASE;
+
+// Doubles of LDS per wave of the forward kernel: its 16-row tile, which during
+// the preamble holds the pencil staging.
+__host__ __device__ constexpr int fwd_wave_region(int nh)
+{
+    return 16 * tile_stride(nh) > FSTG ? 16 * tile_stride(nh) : FSTG;
+}
+
 // Doubles of LDS per wave of the inverse kernel: its 16-row tile, which after
 // the contraction holds the 64 x OSTR output staging of one chain.
 __host__ __device__ constexpr int inv_wave_region(int nc)
@@ -244,6 +257,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
 {
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NH);
+    constexpr int WREG = fwd_wave_region(NH);
     const int B = t.B, N = t.N;
     const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
     const int M = pr.x, Mp = pr.y;
@@ -253,36 +267,77 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int nmf = SMALL ? ((N + 3) >> 2) : 16;  // MFMAs per chain and chunk
 
-    double* tile = sm + w * 16 * TS;
-    double* red = sm + WPP * 16 * TS;   // WPP * 256 (only when WPP > 1)
+    double* tile = sm + w * WREG;
+    double* red = sm + WPP * WREG;   // WPP * 256 (only when WPP > 1)
     double* cf = red + (WPP > 1 ? WPP * 256 : 0);
 
     const int kw = w * 64 * NCH;
 
-    // issue every global load of the preamble before any of them is used
+    // issue every global load of the preamble before any of them is used:
+    // the node tables first (init_chains waits for them), then the pencils
     const int j = lane & 15, kk = lane >> 4;
-    long vec;
-    const LaneCase lc = lane_case(j >> 1, M, Mp, N, &vec);
-    const bool flip = lc.sgn != lc.sgn_odd;
-    const double* vsrc = reinterpret_cast<const double*>(G + vec) + (j & 1);
-    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
-    double sreg[NCH * 16];
-#pragma unroll
-    for (int m = 0; m < NCH * 16; ++m) {
-        const int ks = kw + 4 * m + kk;
-        const int kr = min(flip ? N - 1 - ks : ks, N - 1);
-        sreg[m] = vsrc[2 * max(kr, 0)];
-    }
+    const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
     ChainTables<NCH> ct;
     load_chain_tables<NCH>(ct, t, kw + lane, true);
     const double nf = t.normfac[M * B + Mp];
+    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
+    double sreg[NCH * 16];
+    if (SMALL) {
+        long vec;
+        (void)lane_case(j >> 1, M, Mp, N, &vec);
+        const bool flip = lc.sgn != lc.sgn_odd;
+        const double* vsrc = reinterpret_cast<const double*>(G + vec) + (j & 1);
+#pragma unroll
+        for (int m = 0; m < NCH * 16; ++m) {
+            const int ks = kw + 4 * m + kk;
+            const int kr = min(flip ? N - 1 - ks : ks, N - 1);
+            sreg[m] = vsrc[2 * max(kr, 0)];
+        }
+    }
+    // Non-SMALL: each case's beta-vector is read with whole-wave 16-byte loads
+    // (lane = node, 1 KB contiguous per load, reversed for the d(pi - beta)
+    // cases) and moved into the operand layout through the wave's tile.
+    double2 raw[SMALL ? 1 : NCH][8];
+    int cact = 0;
+    if (!SMALL) {
+#pragma unroll
+        for (int cs = 0; cs < 8; ++cs) {
+            const CaseInfo ci = case_info(cs, M, Mp, N);
+            cact |= ci.active << cs;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const int n = min(kw + 64 * c + lane, N - 1);
+                raw[c][cs] = G[ci.vec + (ci.flip ? N - 1 - n : n)];
+            }
+        }
+    }
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
     Chains<NCH> ch;
     init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+    if (SMALL) {
 #pragma unroll
-    for (int m = 0; m < NCH * 16; ++m)
-        if (!lc.active || kw + 4 * m + kk >= N) sreg[m] = 0.0;
+        for (int m = 0; m < NCH * 16; ++m)
+            if (!lc.active || kw + 4 * m + kk >= N) sreg[m] = 0.0;
+    } else {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const bool live = kw + 64 * c + lane < N;
+            if (c) wave_sync();  // the previous chain's operand reads are done
+#pragma unroll
+            for (int cs = 0; cs < 8; ++cs) {
+                const bool on = live && ((cact >> cs) & 1);
+                double2 v;
+                v.x = on ? raw[c][cs].x : 0.0;
+                v.y = on ? raw[c][cs].y : 0.0;
+                *reinterpret_cast<double2*>(tile + cs * FSTG_CASE + 2 * lane) = v;
+            }
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; ++m) sreg[16 * c + m] = tile[(j >> 1) * FSTG_CASE + 2 * (4 * m + kk) + (j & 1)];
+        }
+        wave_sync();  // operand reads done before the tile is rewritten
+    }
     // output rows of this lane: 4v + lane/16 within each chunk
     const int orow = lane >> 4;
 
@@ -491,7 +546,7 @@ static size_t dwt_smem(int B, int N, bool fwd, int variant)
 {
     DwtShape s = dwt_shape(N, fwd, variant);
     int RP = ((B + 15) / 16) * 16;
-    size_t d = (size_t)s.wpp * (fwd ? 16 * tile_stride(s.nh) : inv_wave_region(s.nch)) + 3 * RP;
+    size_t d = (size_t)s.wpp * (fwd ? fwd_wave_region(s.nh) : inv_wave_region(s.nch)) + 3 * RP;
     if (fwd && s.wpp > 1) d += (size_t)s.wpp * 256;
     return d * sizeof(double);
 }
