@@ -51,6 +51,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIBPATH):
             raise ImportError("pfsoftlib_amd: %s is not built; run __graft_entry__.build()" % _LIBPATH)
+        # One HIP runtime per process: PyTorch bundles its own libamdhip64
+        # (SONAME libamdhip64.so.7, needed by torch as plain libamdhip64.so).
+        # Loaded after this library, torch would map a second runtime next to
+        # the system one and find no GPU; loaded first, it is the runtime this
+        # library binds to.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(_LIBPATH)
         vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
         sig = {

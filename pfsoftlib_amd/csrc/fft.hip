@@ -476,11 +476,115 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
     }
 }
 
+// 512-point lines.  Step A is the register kernel's (32 threads per line, a
+// 16-point DFT each, twiddle, exchange through LDS).  Step B splits each of
+// the 16 DFTs of size 32 over a pair of lanes: lane half h takes the inputs
+// n1 = 2i + h, runs a 16-point DFT, the odd half multiplies by W32^k, and one
+// lane exchange gives X[k] = E0 + W E1 (h = 0) and X[k + 16] = E0 - W E1
+// (h = 1).  Pairs sit 8 lanes apart for tile stores, 16 apart for row stores.
+template <typename T, int KIND, int SIGN>
+__global__ __launch_bounds__((r16_lines<T, KIND>() * 32)) void k_fft_lines_r16x2(const typename CT<T>::C* __restrict__ in,
+                                                                               typename CT<T>::C* __restrict__ out,
+                                                                               LineMap m,
+                                                                               const typename CT<T>::C* __restrict__ tw,
+                                                                               T scale)
+{
+    typedef typename CT<T>::C C;
+    constexpr int TPL = 32;
+    constexpr int L = r16_lines<T, KIND>();
+    constexpr int YS = TPL + 1;
+    constexpr int LS = 16 * YS + 1;
+    extern __shared__ __align__(16) unsigned char smem_r16[];
+
+    const int tid = threadIdx.x;
+    const long u = blockIdx.y, v0 = (long)blockIdx.x * L, sig = blockIdx.z;
+    const C* src = sig_in(in, m, sig) + u * m.in_su + v0 * m.in_sv;
+    C* dst = sig_out(out, m, sig) + u * m.out_su + v0 * m.out_sv;
+
+    // step A
+    int la, n1;
+    if (KIND == LINES_ROW_IN_TILE_OUT) { n1 = tid % TPL; la = tid / TPL; }
+    else { la = tid % L; n1 = tid / L; }
+    C v[16];
+    {
+        const unsigned o = (unsigned)(la * m.in_sv + n1 * m.in_sn), step = (unsigned)(TPL * m.in_sn);
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = gload(src, o + n2 * step);
+    }
+    C w[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) w[k2] = gload(tw, n1 * k2);
+    dft16_perm<SIGN>(v);
+
+    // step B lanes: (line, k2, half)
+    int lb, k2b, h;
+    if (KIND == LINES_ROW_IN_TILE_OUT) { lb = tid % L; const int t = tid / L; k2b = t >> 1; h = t & 1; }
+    else { const int t = tid % TPL; lb = tid / TPL; k2b = t & 15; h = t >> 4; }
+    constexpr int XM = (KIND == LINES_ROW_IN_TILE_OUT) ? L : 16;  // lane distance to the partner half
+    C x[16];
+    if (r16_split<T>()) {
+        T* ly = reinterpret_cast<T*>(smem_r16) + la * LS;
+        const T* lyb = reinterpret_cast<const T*>(smem_r16) + lb * LS + k2b * YS + h;
+        T yi[16];
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            C y = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+            ly[k2 * YS + n1] = y.x;
+            yi[k2] = y.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i].x = lyb[2 * i];
+        __syncthreads();
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) ly[k2 * YS + n1] = yi[k2];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i].y = lyb[2 * i];
+    } else {
+        C* ly = reinterpret_cast<C*>(smem_r16) + la * LS;
+        const C* lyb = reinterpret_cast<const C*>(smem_r16) + lb * LS + k2b * YS + h;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) ly[k2 * YS + n1] = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = lyb[2 * i];
+    }
+    dft16_perm<SIGN>(x);
+    const unsigned o = (unsigned)(lb * m.out_sv + (k2b + 256 * h) * m.out_sn), step = (unsigned)(16 * m.out_sn);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        C e = x[perm_index<16>(k)];
+        const C wk = gload(tw, 16 * k);
+        const C t = twiddle<SIGN>(e, wk);
+        // lane half 0 keeps E0[k] and receives W E1[k]; half 1 the reverse
+        C mine;
+        mine.x = h ? t.x : e.x;
+        mine.y = h ? t.y : e.y;
+        C theirs;
+        theirs.x = __shfl_xor(mine.x, XM);
+        theirs.y = __shfl_xor(mine.y, XM);
+        C r;
+        r.x = (h ? theirs.x - mine.x : mine.x + theirs.x) * scale;
+        r.y = (h ? theirs.y - mine.y : mine.y + theirs.y) * scale;
+        gstore(dst, o + k * step, r);
+    }
+}
+
 template <typename T, int LOGN, int KIND, int SIGN>
 static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m,
                               const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st)
 {
     constexpr int N = 1 << LOGN;
+    if (LOGN == 9) {
+        constexpr int L = r16_lines<T, KIND>();
+        if (m.nv % L != 0) return hipErrorInvalidValue;
+        dim3 grid(m.nv / L, m.nu, nsig);
+        const size_t smem = r16_lds<T, KIND>(LOGN);
+        if (smem > lds_limit()) return hipErrorInvalidConfiguration;
+        pfs::launch((k_fft_lines_r16x2<T, KIND, SIGN>), grid, dim3(L * 32), smem, st, in, out, m, tw, scale);
+        return hipGetLastError();
+    }
     if (LOGN >= 6 && LOGN <= 8) {
         constexpr int L = r16_lines<T, KIND>();
         if (m.nv % L != 0) return hipErrorInvalidValue;

@@ -89,6 +89,27 @@ def test_fft_stage_pencil_layout(ctx, torch, B):
     assert rel(back.cpu().numpy(), g) < 1e-13
 
 
+def test_fft_stage_512_lines_on_slices(ctx, torch):
+    """2B = 512 (B = 256): both FFT stages -- row-in/tile-out and tile-in/row-out
+    512-point lines -- against numpy on three beta-slices of a random grid."""
+    B, N = 256, 512
+    gen = torch.Generator(device="cuda").manual_seed(512)
+    s = torch.cuda.current_stream()
+    g = torch.randn(N ** 3, dtype=torch.complex128, device="cuda", generator=gen)
+    pen = torch.empty_like(g)
+    ctx.stage_fft_fwd(B, g, pen, stream=s)
+    back = torch.empty_like(g)
+    ctx.stage_fft_inv(B, pen, back, stream=s)
+    torch.cuda.synchronize()
+    g3, p3, b3 = g.view(N, N, N), pen.view(N, N, N), back.view(N, N, N)
+    for k in (0, 257, 511):
+        F = np.fft.fft2(g3[k].cpu().numpy())          # [q][p] of slice k
+        assert rel(p3[:, :, k].cpu().numpy(), F.T) < 1e-14
+        P = p3[:, :, k].cpu().numpy().T
+        assert rel(b3[k].cpu().numpy(), np.fft.ifft2(P)) < 1e-14
+    del g, pen, back
+
+
 @pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 19, 20, 32, 33, 64])
 def test_idsoft_matches_oracle(ctx, B):
     c = O.seeded_coeffs(B, 1000 + B)
