@@ -341,8 +341,10 @@ def copy_calibration(torch, steps, mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # steady state: a few round trips leave the GPU clocks still ramping (the
+    # same kernels time 7-10 % slower after 3 warm-up steps than after 10)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--bandwidth", type=int, default=128)
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--power", type=float, default=0.0,

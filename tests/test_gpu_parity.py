@@ -148,14 +148,19 @@ def test_dsoft_b128_matches_oracle(ctx, b128):
 
 @pytest.mark.parametrize("variant", [1, 2, 3])
 def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
-    # every work shape of the FP64 DWT kernels (chains per lane x waves per pair)
+    # every work shape of the FP64 DWT kernels (chains per lane x waves per
+    # pair), at B = 128 and at grids that leave part of the nodes a shape
+    # covers empty (2B = 64, 80, 128)
     B, c, g_ref, c_ref = b128
     ctx.tune("dwt_variant", variant)
     try:
         assert rel(ctx.idsoft(B, c), g_ref) < TOL
         assert rel(ctx.dsoft(B, g_ref), c_ref) < TOL
-        c64 = O.seeded_coeffs(64, 77)
-        assert rel(ctx.idsoft(64, c64), O.idsoft(64, c64)) < TOL
+        for Bs, seed in ((64, 77), (40, 78), (32, 79)):
+            cs = O.seeded_coeffs(Bs, seed)
+            gs = O.idsoft(Bs, cs)
+            assert rel(ctx.idsoft(Bs, cs), gs) < TOL
+            assert rel(ctx.dsoft(Bs, gs), O.dsoft(Bs, gs)) < TOL
     finally:
         ctx.tune("dwt_variant", 0)
 
