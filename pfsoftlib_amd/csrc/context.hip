@@ -564,7 +564,10 @@ int dsoft_dev(pfs_context* c, int B, const double2* grid, double2* coeffs, hipSt
     return PFS_OK;
 }
 
-int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipStream_t st)
+// stale: the packed Nyquist entries of the output grid (see
+// k_nyquist_gather_f64), or null to read them from grid itself.
+int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipStream_t st,
+               const double2* stale = nullptr)
 {
     Plan* p;
     int rc = get_plan(c, B, &p);
@@ -573,7 +576,7 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     if (rc) return rc;
     double2* W = (double2*)c->work[0];
     double2* G = (double2*)c->work[1];
-    HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st); }));
+    HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st, stale); }));
     HIP_TRY(timed(c, "dwt_inv", st, [&] { return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st, (int)c->dwt_variant); }));
     return fft_inv(c, *p, G, W, grid, st);
 }
@@ -1023,8 +1026,25 @@ int pfs_idsoft_f64(pfs_context* c, int B, const double* coeffs, double* grid)
     rc = ensure_io(c, grid_bytes(B), coeff_bytes(B));
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(c->io_coeff, coeffs, coeff_bytes(B), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->io_grid, grid, grid_bytes(B), hipMemcpyHostToDevice, c->stream));
-    rc = idsoft_dev(c, B, (const double2*)c->io_coeff, (double2*)c->io_grid, c->stream);
+    // Of the caller's grid only the stale Nyquist entries are read (the
+    // reference transforms in place: src/fn_idsoft.cpp:100-283 never writes
+    // them, grid3D_cx.hpp:393-427 then reads them).  They go up packed, into
+    // the front of the output buffer, which the last FFT pass overwrites after
+    // they have been read: 2 MB instead of the 268 MB grid at B = 128.
+    const long N = 2L * B;
+    std::vector<double2> stale((size_t)(2 * N - 1) * N);
+    const double2* hg = reinterpret_cast<const double2*>(grid);
+    for (long k = 0; k < N; ++k) {
+        const double2* slice = hg + N * N * k;
+        for (long e = 0; e < 2 * N - 1; ++e) {
+            long p, q;
+            if (e < N) { p = B; q = e; }
+            else { q = B; p = e - N; if (p >= B) ++p; }
+            stale[(size_t)(e * N + k)] = slice[N * q + p];
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(c->io_grid, stale.data(), stale.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    rc = idsoft_dev(c, B, (const double2*)c->io_coeff, (double2*)c->io_grid, c->stream, (const double2*)c->io_grid);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(grid, c->io_grid, grid_bytes(B), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

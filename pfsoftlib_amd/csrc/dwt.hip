@@ -600,7 +600,12 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
 }
 
 // G[p][q][k] <- grid(row p, col q, slice k) for p == B or q == B.
-__global__ void k_nyquist_gather_f64(const double2* __restrict__ grid, double2* __restrict__ G, int B)
+// The (2N - 1) N stale Nyquist entries -- spectral row p = B and column q = B
+// of every slice -- either read from the caller's grid or, when compact is
+// set, from a packed copy [e][k] (e < N: (p, q) = (B, e); e >= N: q = B,
+// p = e - N, skipping p = B) that a host caller uploads instead of the grid.
+__global__ void k_nyquist_gather_f64(const double2* __restrict__ grid, double2* __restrict__ G, int B,
+                                     const double2* __restrict__ compact)
 {
     const int N = 2 * B;
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -610,14 +615,14 @@ __global__ void k_nyquist_gather_f64(const double2* __restrict__ grid, double2* 
     int p, q;
     if (e < N) { p = B; q = e; }
     else { q = B; p = e - N; if (p >= B) ++p; }
-    G[((long)p * N + q) * N + k] = grid[(long)N * N * k + (long)N * q + p];
+    G[((long)p * N + q) * N + k] = compact ? compact[idx] : grid[(long)N * N * k + (long)N * q + p];
 }
 
-hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st)
+hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st, const double2* compact)
 {
     const int N = 2 * B;
     long total = (long)(2 * N - 1) * N;
-    pfs::launch(k_nyquist_gather_f64, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, grid, G, B);
+    pfs::launch(k_nyquist_gather_f64, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, grid, G, B, compact);
     return hipGetLastError();
 }
 

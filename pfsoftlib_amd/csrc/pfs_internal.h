@@ -184,7 +184,8 @@ hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st);
 
 // Copy the grid entries the inverse DWT never writes (row == B or col == B)
 // into the pencil buffer, so the inverse FFT reads the caller's values there.
-hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st);
+hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st,
+                                     const double2* compact = nullptr);
 
 // ---- batched pipeline entry points used by the request server ------------
 }  // namespace pfs
