@@ -16,6 +16,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <thread>
 #include <cstdio>
 #include <cstdarg>
 #include <cstring>
@@ -247,6 +248,10 @@ struct pfs_context {
     size_t io_grid_bytes = 0;
     void* io_coeff = nullptr;
     size_t io_coeff_bytes = 0;
+    // pinned staging for the host-buffer entry points: two chunks, each with
+    // the event of the last copy that used it
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // allocations handed out by pfs_device_alloc
     std::map<void*, size_t> user_allocs;
     // per-kernel timing: a start/stop event pair recorded around every launch
@@ -444,6 +449,89 @@ int ensure_io(pfs_context* c, size_t gbytes, size_t cbytes)
         int rc = ctx_alloc(c, &c->io_coeff, cbytes);
         if (rc) return rc;
         c->io_coeff_bytes = cbytes;
+    }
+    return PFS_OK;
+}
+
+// ---- host <-> device copies through pinned staging ------------------------
+// A pageable copy goes through the runtime's own single-threaded staging.  The
+// host-buffer entry points instead copy chunks of the caller's buffer into two
+// pinned buffers with several threads and overlap that with the DMA of the
+// other buffer.
+constexpr size_t STAGE_BYTES = 32u << 20;
+
+int ensure_staging(pfs_context* c)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (!c->stage[i] && hipHostMalloc(&c->stage[i], STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+            c->stage[i] = nullptr;
+            return fail(PFS_ERR_MEMORY, "hipHostMalloc(%zu) for pinned staging failed", STAGE_BYTES);
+        }
+        if (!c->stage_ev[i] && hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming) != hipSuccess) {
+            c->stage_ev[i] = nullptr;
+            return fail(PFS_ERR_DEVICE, "hipEventCreate for pinned staging failed");
+        }
+    }
+    return PFS_OK;
+}
+
+// memcpy split over a few threads (one host thread streams ~10 GB/s)
+void par_memcpy(void* dst, const void* src, size_t n)
+{
+    const size_t min_part = 4u << 20;
+    unsigned nt = std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, n / min_part));
+    if (nt <= 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n / nt + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) {
+        const size_t off = t * part;
+        if (off >= n) break;
+        th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); });
+    }
+    std::memcpy(dst, src, std::min(part, n));
+    for (auto& x : th) x.join();
+}
+
+int copy_h2d(pfs_context* c, void* dev, const void* host, size_t n, hipStream_t st)
+{
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    // the host writes the staging buffers: no earlier copy may still read them
+    for (int b = 0; b < 2; ++b) HIP_TRY(hipEventSynchronize(c->stage_ev[b]));
+    for (size_t off = 0, i = 0; off < n; off += STAGE_BYTES, ++i) {
+        const int b = (int)(i & 1);
+        const size_t len = std::min(STAGE_BYTES, n - off);
+        if (i >= 2) HIP_TRY(hipEventSynchronize(c->stage_ev[b]));  // its last DMA is done
+        par_memcpy(c->stage[b], (const char*)host + off, len);
+        HIP_TRY(hipMemcpyAsync((char*)dev + off, c->stage[b], len, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(c->stage_ev[b], st));
+    }
+    return PFS_OK;
+}
+
+// Returns once the data is in host memory.
+int copy_d2h(pfs_context* c, void* host, const void* dev, size_t n, hipStream_t st)
+{
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    const size_t nchunk = (n + STAGE_BYTES - 1) / STAGE_BYTES;
+    auto issue = [&](size_t i) -> hipError_t {
+        const int b = (int)(i & 1);
+        const size_t off = i * STAGE_BYTES, len = std::min(STAGE_BYTES, n - off);
+        hipError_t e = hipMemcpyAsync(c->stage[b], (const char*)dev + off, len, hipMemcpyDeviceToHost, st);
+        return e == hipSuccess ? hipEventRecord(c->stage_ev[b], st) : e;
+    };
+    for (size_t i = 0; i < std::min<size_t>(2, nchunk); ++i) HIP_TRY(issue(i));
+    for (size_t i = 0; i < nchunk; ++i) {
+        const int b = (int)(i & 1);
+        const size_t off = i * STAGE_BYTES, len = std::min(STAGE_BYTES, n - off);
+        HIP_TRY(hipEventSynchronize(c->stage_ev[b]));
+        par_memcpy((char*)host + off, c->stage[b], len);
+        if (i + 2 < nchunk) HIP_TRY(issue(i + 2));
     }
     return PFS_OK;
 }
@@ -964,6 +1052,12 @@ int pfs_ctx_destroy(pfs_context* c)
         ctx_free(c, c->work[1], c->work_bytes);
         ctx_free(c, c->io_grid, c->io_grid_bytes);
         ctx_free(c, c->io_coeff, c->io_coeff_bytes);
+        for (int i = 0; i < 2; ++i) {
+            if (c->stage[i]) (void)hipHostFree(c->stage[i]);
+            if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+            c->stage[i] = nullptr;
+            c->stage_ev[i] = nullptr;
+        }
         for (auto& kv : c->user_allocs) {
             void* ptr = kv.first;
             ctx_free(c, ptr, kv.second);
@@ -1008,12 +1102,11 @@ int pfs_dsoft_f64(pfs_context* c, int B, const double* grid, double* coeffs)
     HIP_TRY(hipSetDevice(c->device));
     rc = ensure_io(c, grid_bytes(B), coeff_bytes(B));
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(c->io_grid, grid, grid_bytes(B), hipMemcpyHostToDevice, c->stream));
+    rc = copy_h2d(c, c->io_grid, grid, grid_bytes(B), c->stream);
+    if (rc) return rc;
     rc = dsoft_dev(c, B, (const double2*)c->io_grid, (double2*)c->io_coeff, c->stream);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(coeffs, c->io_coeff, coeff_bytes(B), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return PFS_OK;
+    return copy_d2h(c, coeffs, c->io_coeff, coeff_bytes(B), c->stream);
 }
 
 int pfs_idsoft_f64(pfs_context* c, int B, const double* coeffs, double* grid)
@@ -1025,7 +1118,8 @@ int pfs_idsoft_f64(pfs_context* c, int B, const double* coeffs, double* grid)
     HIP_TRY(hipSetDevice(c->device));
     rc = ensure_io(c, grid_bytes(B), coeff_bytes(B));
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(c->io_coeff, coeffs, coeff_bytes(B), hipMemcpyHostToDevice, c->stream));
+    rc = copy_h2d(c, c->io_coeff, coeffs, coeff_bytes(B), c->stream);
+    if (rc) return rc;
     // Of the caller's grid only the stale Nyquist entries are read (the
     // reference transforms in place: src/fn_idsoft.cpp:100-283 never writes
     // them, grid3D_cx.hpp:393-427 then reads them).  They go up packed, into
@@ -1046,9 +1140,7 @@ int pfs_idsoft_f64(pfs_context* c, int B, const double* coeffs, double* grid)
     HIP_TRY(hipMemcpyAsync(c->io_grid, stale.data(), stale.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     rc = idsoft_dev(c, B, (const double2*)c->io_coeff, (double2*)c->io_grid, c->stream, (const double2*)c->io_grid);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(grid, c->io_grid, grid_bytes(B), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return PFS_OK;
+    return copy_d2h(c, grid, c->io_grid, grid_bytes(B), c->stream);
 }
 
 int pfs_dsoft_f64_dev(pfs_context* c, int B, const void* grid_dev, void* coeffs_dev, void* stream)
