@@ -989,13 +989,12 @@ int seam_dft2(int cols, int rows, int lays, double* arr, int sign)
     if (rc) return rc;
     rc = ensure_work(c, bytes);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(c->io_grid, arr, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipSetDevice(c->device));
+    rc = copy_h2d(c, c->io_grid, arr, bytes, c->stream);
+    if (rc) return rc;
     rc = dft2_dev(c, cols, lays, c->io_grid, sign, 1.0, c->stream);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(arr, c->io_grid, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return PFS_OK;
+    return copy_d2h(c, arr, c->io_grid, bytes, c->stream);
 }
 
 }  // namespace

@@ -157,6 +157,12 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int ts
 // 16 rows an MFMA operand read touches fall in distinct banks).
 __host__ __device__ constexpr int tile_stride(int nc) { return 64 * nc + 1; }
 
+// Pencil staging of the forward preamble: 8 cases x 64 nodes x (re, im) for
+// one chain, case rows padded to 136 doubles (272 words = 16 mod 64) so the
+// operand-layout reads spread over all banks.
+constexpr int FSTG_CASE = 136;
+constexpr int FSTG = 8 * FSTG_CASE;
+
 // Per-lane view of one case of the pair (hoisted out of the degree loop).
 struct LaneCase {
     int Mc, Mpc, active;
@@ -200,25 +206,64 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
     double* red = sm + WPP * 16 * TS;   // WPP * 256 (only when WPP > 1)
     double* cf = red + (WPP > 1 ? WPP * 256 : 0);
 
-    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
-
     const int kw = w * 64 * NCH;
-    Chains<NCH> ch;
-    init_chains<NCH>(ch, t, M, Mp, kw + lane, true);
-
-    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
     const int j = lane & 15, kk = lane >> 4;
     long vec;
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, &vec);
-    const bool flip = lc.sgn != lc.sgn_odd;
-    const double* vsrc = reinterpret_cast<const double*>(G + vec) + (j & 1);
-    double sreg[NCH * 16];
+    // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im.
+    // With room in the tile, each case's beta-vector is read with whole-wave
+    // 16-byte loads (lane = node, 1 KB contiguous per load, reversed for the
+    // d(pi - beta) cases), issued before the rest of the preamble, and moved
+    // into the operand layout through the tile.
+    constexpr bool STAGED = !SMALL && 16 * TS >= FSTG;
+    double2 raw[STAGED ? NCH : 1][8];
+    int cact = 0;
+    if (STAGED) {
 #pragma unroll
-    for (int m = 0; m < NCH * 16; ++m) {
-        int ks = kw + 4 * m + kk;
-        double v = 0.0;
-        if (lc.active && ks < N) v = vsrc[2 * (flip ? N - 1 - ks : ks)];
-        sreg[m] = v;
+        for (int cs = 0; cs < 8; ++cs) {
+            const CaseInfo ci = case_info(cs, M, Mp, N);
+            cact |= ci.active << cs;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const int n = min(kw + 64 * c + lane, N - 1);
+                raw[c][cs] = G[ci.vec + (ci.flip ? N - 1 - n : n)];
+            }
+        }
+    }
+
+    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, t, M, Mp, kw + lane, true);
+
+    double sreg[NCH * 16];
+    if (STAGED) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const bool live = kw + 64 * c + lane < N;
+            if (c) wave_sync();  // the previous chain's operand reads are done
+#pragma unroll
+            for (int cs = 0; cs < 8; ++cs) {
+                const bool on = live && ((cact >> cs) & 1);
+                double2 v;
+                v.x = on ? raw[c][cs].x : 0.0;
+                v.y = on ? raw[c][cs].y : 0.0;
+                *reinterpret_cast<double2*>(tile + cs * FSTG_CASE + 2 * lane) = v;
+            }
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; ++m) sreg[16 * c + m] = tile[(j >> 1) * FSTG_CASE + 2 * (4 * m + kk) + (j & 1)];
+        }
+        wave_sync();  // operand reads done before the tile is rewritten
+    } else {
+        const bool flip = lc.sgn != lc.sgn_odd;
+        const double* vsrc = reinterpret_cast<const double*>(G + vec) + (j & 1);
+#pragma unroll
+        for (int m = 0; m < NCH * 16; ++m) {
+            int ks = kw + 4 * m + kk;
+            double v = 0.0;
+            if (lc.active && ks < N) v = vsrc[2 * (flip ? N - 1 - ks : ks)];
+            sreg[m] = v;
+        }
     }
     // output rows of this lane: 4v + lane/16 within each chunk
     const int orow = lane >> 4;
