@@ -171,6 +171,21 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// The forward reads each pencil once: a non-temporal load does not allocate in
+// the caches, so it neither evicts the pencils the FFT's last pass left in the
+// Infinity Cache nor the coefficient lines neighbouring pairs share in L2.
+// (Pencil stores of the inverse stay ordinary: the next FFT pass reads what
+// they leave in the Infinity Cache.)
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 load_nt(const double2* p)
+{
+    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    double2 r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+
 // Broadcast lane `src` (uniform) of a double to every lane, via SGPRs.
 __device__ __forceinline__ double bcast_lane(double v, int src)
 {
@@ -307,7 +322,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const int n = min(kw + 64 * c + lane, N - 1);
-                raw[c][cs] = G[ci.vec + (ci.flip ? N - 1 - n : n)];
+                raw[c][cs] = load_nt(G + ci.vec + (ci.flip ? N - 1 - n : n));
             }
         }
     }

@@ -162,6 +162,7 @@ __host__ __device__ constexpr int tile_stride(int nc) { return 64 * nc + 1; }
 // operand-layout reads spread over all banks.
 constexpr int FSTG_CASE = 136;
 constexpr int FSTG = 8 * FSTG_CASE;
+typedef double d2v __attribute__((ext_vector_type(2)));
 
 // Per-lane view of one case of the pair (hoisted out of the degree loop).
 struct LaneCase {
@@ -226,7 +227,10 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const int n = min(kw + 64 * c + lane, N - 1);
-                raw[c][cs] = G[ci.vec + (ci.flip ? N - 1 - n : n)];
+                const double2* src = G + ci.vec + (ci.flip ? N - 1 - n : n);
+                const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(src));  // read once
+                raw[c][cs].x = v.x;
+                raw[c][cs].y = v.y;
             }
         }
     }
