@@ -487,12 +487,19 @@ void par_memcpy(void* dst, const void* src, size_t n)
     }
     const size_t part = (n / nt + 63) & ~(size_t)63;
     std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt; ++t) {
-        const size_t off = t * part;
-        if (off >= n) break;
-        th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); });
+    size_t done = part;  // [0, done) is this thread's, the rest the workers' or, failing them, this thread's
+    try {
+        for (unsigned t = 1; t < nt; ++t) {
+            const size_t off = t * part;
+            if (off >= n) break;
+            th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); });
+            done = std::min(n, off + part);
+        }
+    } catch (...) {
+        // no more threads: copy what is left here (no exception crosses the C ABI)
     }
     std::memcpy(dst, src, std::min(part, n));
+    if (done < n) std::memcpy((char*)dst + done, (const char*)src + done, n - done);
     for (auto& x : th) x.join();
 }
 
