@@ -327,6 +327,41 @@ def serving_leg(torch, P, ctx, seconds, seed, budget_bytes, slots, max_batch, ut
     }
 
 
+def dwt_accuracy(ctx, np, max_bw, runs, seed):
+    """DWT round-trip accuracy on the device's Wigner-d tables, as the reference's
+    benchmark/benchmark_dwt_accuracy.cpp measures it: for BW = 2, 4, ..., max_bw and
+    (M, M') = (0, 0), (BW/2, 0), (BW/2, BW/2), random f in [-1, 1] + i[-1, 1], the
+    inverse s = d^T f and the forward g = d_w s, and the mean over runs of
+    |g - f| / |f| and |g - f|.  Corrected: the reference builds the unweighted
+    (0, 0) table into dw at :60, so its dt stays zero there."""
+    rng = np.random.default_rng(seed)
+    cases = [("M=0,M'=0", lambda bw: (0, 0)), ("M=BW/2,M'=0", lambda bw: (bw // 2, 0)),
+             ("M=M'=BW/2", lambda bw: (bw // 2, bw // 2))]
+    out = {"cases": [c[0] for c in cases], "runs": runs, "bandwidths": [], "relative": [], "absolute": [],
+           "note": "device tables (pfs_wigner_d_matrix); the reference's table mix-up at "
+                   "benchmark_dwt_accuracy.cpp:60 corrected"}
+    bw = 2
+    while bw <= max_bw:
+        rel_row, abs_row = [], []
+        for _, mm in cases:
+            M, Mp = mm(bw)
+            dw = ctx.wigner_d_matrix(bw, M, Mp, True)
+            dt = ctx.wigner_d_matrix(bw, M, Mp, False)
+            r = a = 0.0
+            for _ in range(runs):
+                f = rng.uniform(-1, 1, dw.shape[0]) + 1j * rng.uniform(-1, 1, dw.shape[0])
+                d = float(np.linalg.norm(dw @ (dt.T @ f) - f))
+                r += d / float(np.linalg.norm(f))
+                a += d
+            rel_row.append(r / runs)
+            abs_row.append(a / runs)
+        out["bandwidths"].append(bw)
+        out["relative"].append(rel_row)
+        out["absolute"].append(abs_row)
+        bw *= 2
+    return out
+
+
 def copy_calibration(torch, steps, mib):
     """A plain elementwise copy of known size, for scaling PMC byte counters."""
     n = mib * (1 << 20) // 8
@@ -486,6 +521,7 @@ def main():
     if args.serve_seconds > 0:
         out["serving"] = serving_leg(torch, P, ctx, args.serve_seconds, args.seed, int(args.serve_budget_gb * 2 ** 30),
                                      args.serve_slots, args.serve_max_batch, rate=args.serve_rate or None)
+    out["dwt_accuracy"] = dwt_accuracy(ctx, np, 256, 10, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
     print(json.dumps(out))

@@ -58,6 +58,18 @@ def test_wigner_table_on_device(ctx, B, M, Mp, weighted):
     assert np.linalg.norm(d - ref) / np.linalg.norm(ref) < 1e-12
 
 
+@pytest.mark.parametrize("bw", [2, 16, 128, 256])
+def test_dwt_accuracy_roundtrip_on_device_tables(ctx, bw):
+    # benchmark/benchmark_dwt_accuracy.cpp's round trip f -> d_w (d^T f) on the
+    # device tables (with its :60 table mix-up corrected): identity to rounding
+    rng = np.random.default_rng(bw)
+    for M, Mp in [(0, 0), (bw // 2, 0), (bw // 2, bw // 2)]:
+        dw = ctx.wigner_d_matrix(bw, M, Mp, True)
+        dt = ctx.wigner_d_matrix(bw, M, Mp, False)
+        f = rng.uniform(-1, 1, dw.shape[0]) + 1j * rng.uniform(-1, 1, dw.shape[0])
+        assert rel(dw @ (dt.T @ f), f) < 1e-12
+
+
 @pytest.mark.parametrize("n,lays", [(2, 2), (4, 3), (6, 4), (8, 8), (12, 5), (16, 16), (20, 3), (64, 8),
                                     (256, 4), (512, 2)])
 @pytest.mark.parametrize("inverse", [False, True])
