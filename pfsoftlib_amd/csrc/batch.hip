@@ -208,24 +208,32 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
         for (int sp = 0; sp < SPW; ++sp) acc[ch][sp] = (v4f){0.f, 0.f, 0.f, 0.f};
 
     const bool odd = lane & 1, flip = cb.flip;
-    FwdStage<NN, SPW, NC> S0, S1;
-    fwd_load(S0, gp, tf, wl, e0, de, 0);
-    if constexpr (NKB == 1) {
-        fwd_mfma(acc, S0, odd, flip);
+    // ST register stages: block kb + ST - 1 loads while block kb contracts.  A
+    // pair with one or two chunks does only 8 SPW or 16 SPW MFMAs per block,
+    // too little to cover a load's latency with one block in flight.
+    constexpr int ST0 = NC <= 2 ? 4 : 2;
+    constexpr int ST = ST0 < NKB ? ST0 : NKB;
+    FwdStage<NN, SPW, NC> S[ST];
+#pragma unroll
+    for (int st = 0; st + 1 < ST; ++st) fwd_load(S[st], gp, tf, wl, e0, de, st);
+    if constexpr (ST == 1) {
+        fwd_load(S[0], gp, tf, wl, e0, de, 0);
+        fwd_mfma(acc, S[0], odd, flip);
     } else {
         // The scheduling barriers keep each stage's loads ahead of the other
-        // stage's MFMAs (left alone, the scheduler sinks them next to their use).
-        for (int kb = 0; kb < NKB; kb += 2) {
-            fwd_load(S1, gp, tf, wl, e0, de, kb + 1);
-            __builtin_amdgcn_sched_barrier(0);
-            fwd_mfma(acc, S0, odd, flip);
-            __builtin_amdgcn_sched_barrier(0);
-            // unconditional (the last one re-reads block kb): a branch here would
-            // make the compiler drain every outstanding load before the next MFMAs
-            fwd_load(S0, gp, tf, wl, e0, de, kb + 2 < NKB ? kb + 2 : kb);
-            __builtin_amdgcn_sched_barrier(0);
-            fwd_mfma(acc, S1, odd, flip);
-            __builtin_amdgcn_sched_barrier(0);
+        // stages' MFMAs (left alone, the scheduler sinks them next to their use).
+        for (int kb = 0; kb < NKB; kb += ST) {
+#pragma unroll
+            for (int st = 0; st < ST; ++st) {
+                // unconditional (past the end it re-reads the last block): a
+                // branch here would make the compiler drain every outstanding
+                // load before the next MFMAs
+                const int nk = kb + st + ST - 1;
+                fwd_load(S[(st + ST - 1) % ST], gp, tf, wl, e0, de, nk < NKB ? nk : NKB - 1);
+                __builtin_amdgcn_sched_barrier(0);
+                fwd_mfma(acc, S[st], odd, flip);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
 
