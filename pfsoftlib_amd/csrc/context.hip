@@ -269,7 +269,7 @@ struct pfs_context {
     long server_autotune = 1;   // request servers autotune the batched shapes at creation
     long batch_force_shape = 0; // 0: tuned / default shapes; else 10 * waves + signals per wave
     long batch_fft_signals = 0; // batched FFT: signals per chunk (0 = the whole sub-batch)
-    long batch_fft_slices = 0;  // batched FFT: beta-slices per chunk (0 = all); a multiple of 16
+    long batch_fft_slices = 0;  // batched FFT: beta-slices per chunk (0 = all); a multiple of 32
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -860,7 +860,8 @@ int batch_fft_signals(const pfs_context* c, int n)
 int batch_fft_slices(const pfs_context* c, int N)
 {
     const long k = c->batch_fft_slices;
-    return (k >= 16 && k < N && N % k == 0) ? (int)k : N;
+    // the complex64 register passes move 32 slices per workgroup
+    return (k >= 32 && k < N && N % k == 0) ? (int)k : N;
 }
 
 // Forward batch of n <= cap signals: FFT rows (grids -> W), FFT columns
@@ -1328,7 +1329,7 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         return PFS_OK;
     }
     if (!strcmp(key, "batch_fft_slices")) {
-        if (value < 0 || (value && (value % 16))) return fail(PFS_ERR_ARG, "batch_fft_slices must be 0 or a multiple of 16");
+        if (value < 0 || (value && (value % 32))) return fail(PFS_ERR_ARG, "batch_fft_slices must be 0 or a multiple of 32");
         c->batch_fft_slices = value;
         return PFS_OK;
     }

@@ -221,12 +221,13 @@ template <typename C> __device__ __forceinline__ C gload(const C* base, unsigned
     r.y = v.y;
     return r;
 }
-template <typename C> __device__ __forceinline__ void gstore(C* base, unsigned i, C r)
+template <typename C, bool NT = false> __device__ __forceinline__ void gstore(C* base, unsigned i, C r)
 {
     typename GV<C>::T v;
     v.x = r.x;
     v.y = r.y;
-    ((gvec<C>*)base)[i] = v;
+    if constexpr (NT) __builtin_nontemporal_store(v, (gvec<C>*)base + i);
+    else ((gvec<C>*)base)[i] = v;
 }
 // Grid data is read once per pass with non-temporal (streaming) loads: they
 // do not allocate in the caches, so the part of the input that the previous
@@ -370,11 +371,23 @@ template <int Rn, int SIGN, typename C> __device__ __forceinline__ void dft_smal
 }
 
 // Lines per workgroup of the register kernel: the tile-side accesses move
-// whole runs of one column element over the workgroup's lines.
+// whole runs of one column element over the workgroup's lines.  Measured on
+// MI355X at 2B = 256: complex128 tile writes prefer 8 lines (128-byte runs),
+// complex128 tile reads 16 (256 bytes); complex64 passes, 64 grids per launch,
+// run 4-9 % faster at 32 lines (256 bytes) than at 16.
 template <typename T, int KIND> constexpr int r16_lines()
 {
-    // measured on MI355X at 2B = 256: complex128 tile writes prefer 8 lines
-    return (sizeof(T) == 8 && KIND == LINES_ROW_IN_TILE_OUT) ? 8 : 16;
+    if (sizeof(T) == 4) return 32;
+    return KIND == LINES_ROW_IN_TILE_OUT ? 8 : 16;
+}
+// Tile stores of complex64 passes are non-temporal.  Those passes stream many
+// grids (8.6 GB for 64 grids at 2B = 256), far beyond the Infinity Cache, and
+// measured 3-9 % faster so; the complex64 row stores (inverse passes) and all
+// complex128 stores stay ordinary (for one complex128 grid the next kernel
+// reads part of the output from the Infinity Cache).
+template <typename T, int KIND> constexpr bool r16_nt_store()
+{
+    return sizeof(T) == 4 && KIND == LINES_ROW_IN_TILE_OUT;
 }
 // y * w for a twiddle w = tw[e] of the forward table exp(-2 pi i e / N),
 // conjugated for SIGN > 0.  tw[0] = 1 exactly, so e = 0 needs no branch (a
@@ -485,7 +498,7 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
             C r = x[q][perm_index<TPL>(k1)];
             r.x *= scale;
             r.y *= scale;
-            gstore(dst, o + k1 * step, r);
+            gstore<C, r16_nt_store<T, KIND>()>(dst, o + k1 * step, r);
         }
     }
 }
@@ -581,7 +594,7 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * 32)) void k_fft_lines_r16x2
         C r;
         r.x = (h ? theirs.x - mine.x : mine.x + theirs.x) * scale;
         r.y = (h ? theirs.y - mine.y : mine.y + theirs.y) * scale;
-        gstore(dst, o + k * step, r);
+        gstore<C, r16_nt_store<T, KIND>()>(dst, o + k * step, r);
     }
 }
 

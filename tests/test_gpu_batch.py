@@ -147,7 +147,7 @@ def test_chunked_batch_fft_is_bitwise_invariant(ctx, torch, B, nsig):
     cs = np.concatenate([O.seeded_coeffs(B, 80 + i) for i in range(nsig)]).astype(np.complex64)
     cdev = torch.from_numpy(cs).cuda()
     outs = []
-    for S, K in [(0, 0), (2, 16), (3, 32), (1, 0), (0, 64)]:
+    for S, K in [(0, 0), (2, 32), (3, 64), (1, 0), (0, 64)]:
         ctx.tune("batch_fft_signals", S)
         ctx.tune("batch_fft_slices", K)
         g = torch.empty(nsig * N ** 3, dtype=torch.complex64, device="cuda")
