@@ -186,18 +186,43 @@ __device__ __forceinline__ double2 load_nt(const double2* p)
     return r;
 }
 
-// Broadcast lane `src` (uniform) of a double to every lane, via SGPRs.
-__device__ __forceinline__ double bcast_lane(double v, int src)
+// Lane rr of each 16-lane row of v, in every lane of that row: one DPP move
+// (row_newbcast:rr) into a VGPR.  Two v_readlane into SGPRs cost twice the
+// VALU issue, and a VALU op may read only one SGPR pair, so the SGPR form
+// also needed a copy back to a VGPR per step (measured: 116.8 -> 112.6 us
+// forward, 125.7 -> 121.8 us inverse at B = 128).
+template <int RR> __device__ __forceinline__ double row_bcast_c(double v)
 {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-    return __hiloint2double(hi, lo);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + RR, 0xf, 0xf, false);
+}
+// rr is a constant once the caller's loop is unrolled; the switch folds away.
+__device__ __forceinline__ double row_bcast(double v, int rr)
+{
+    switch (rr) {
+    case 0: return row_bcast_c<0>(v);
+    case 1: return row_bcast_c<1>(v);
+    case 2: return row_bcast_c<2>(v);
+    case 3: return row_bcast_c<3>(v);
+    case 4: return row_bcast_c<4>(v);
+    case 5: return row_bcast_c<5>(v);
+    case 6: return row_bcast_c<6>(v);
+    case 7: return row_bcast_c<7>(v);
+    case 8: return row_bcast_c<8>(v);
+    case 9: return row_bcast_c<9>(v);
+    case 10: return row_bcast_c<10>(v);
+    case 11: return row_bcast_c<11>(v);
+    case 12: return row_bcast_c<12>(v);
+    case 13: return row_bcast_c<13>(v);
+    case 14: return row_bcast_c<14>(v);
+    default: return row_bcast_c<15>(v);
+    }
 }
 
 // Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
-// column 64 (c - C0) + lane) and advances them.  Lanes 0..15 fetch the step
-// coefficients of the 16 degrees in one LDS read each; every step then takes
-// them from SGPRs, so no LDS round trip sits on the recurrence chain.
+// column 64 (c - C0) + lane) and advances them.  Lane L fetches the step
+// coefficients of degree r0 + L % 16 in one LDS read each, so every 16-lane
+// row holds all 16; each step then broadcasts its own within the rows, and no
+// LDS round trip sits on the recurrence chain.
 template <int NCH, int C0, int NC>
 __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__ tile, int tstr, int lane,
                                            const double* __restrict__ cf, int RP, int r0)
@@ -206,7 +231,7 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__
     const double vf1 = cf[rl], vf1f2 = cf[RP + rl], vc1 = cf[2 * RP + rl];
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
-        const double f1 = bcast_lane(vf1, rr), f1f2 = bcast_lane(vf1f2, rr), c1 = bcast_lane(vc1, rr);
+        const double f1 = row_bcast(vf1, rr), f1f2 = row_bcast(vf1f2, rr), c1 = row_bcast(vc1, rr);
 #pragma unroll
         for (int c = C0; c < C0 + NC; ++c) {
             tile[rr * tstr + 64 * (c - C0) + lane] = ch.cur[c];
