@@ -31,12 +31,14 @@ def algorithmic_bytes_per_step(kernel, B, e=16):
     FFT stage per direction: 2 N^3 e (one read + one write of the grid),
     split evenly between its row and column passes.
     DWT stage per direction: (N-1)^2 N e pencil bytes + B(4B^2-1)/3 e
-    coefficient bytes."""
+    coefficient bytes; up to N = 256 the inverse DWT launch also copies the
+    (2N-1) N stale Nyquist entries (one read and one write each)."""
     N = 2 * B
     if kernel.startswith("fft_"):
         return N ** 3 * e
     if kernel.startswith("dwt_"):
-        return (N - 1) ** 2 * N * e + B * (4 * B * B - 1) // 3 * e
+        nyq = 2 * (2 * N - 1) * N * e if kernel == "dwt_inv" and N <= 256 else 0
+        return (N - 1) ** 2 * N * e + B * (4 * B * B - 1) // 3 * e + nyq
     if kernel == "nyquist_gather":
         return 2 * (2 * N - 1) * N * e
     return 0

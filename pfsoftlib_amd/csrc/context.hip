@@ -671,8 +671,13 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     if (rc) return rc;
     double2* W = (double2*)c->work[0];
     double2* G = (double2*)c->work[1];
-    HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st, stale); }));
-    HIP_TRY(timed(c, "dwt_inv", st, [&] { return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st, (int)c->dwt_variant); }));
+    // up to 2B = 256 the inverse DWT launch copies the stale Nyquist entries itself
+    const bool fused = p->t.N <= 256;
+    if (!fused) HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st, stale); }));
+    HIP_TRY(timed(c, "dwt_inv", st, [&] {
+        return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st, (int)c->dwt_variant, fused && !stale ? grid : nullptr,
+                                  fused ? stale : nullptr);
+    }));
     return fft_inv(c, *p, G, W, grid, st);
 }
 

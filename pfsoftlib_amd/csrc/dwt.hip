@@ -430,11 +430,38 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
-// Inverse DWT: every wave owns its nodes outright (no reduction).
+// G[p][q][k] <- grid(row p, col q, slice k) for p == B or q == B.
+// The (2N - 1) N stale Nyquist entries -- spectral row p = B and column q = B
+// of every slice -- either read from the caller's grid or, when compact is
+// set, from a packed copy [e][k] (e < N: (p, q) = (B, e); e >= N: q = B,
+// p = e - N, skipping p = B) that a host caller uploads instead of the grid.
+__device__ __forceinline__ void nyquist_gather_one(const double2* __restrict__ grid, double2* __restrict__ G, int B,
+                                                   const double2* __restrict__ compact, long idx)
+{
+    const int N = 2 * B;
+    const long total = (long)(2 * N - 1) * N;
+    if (idx >= total) return;
+    int e = (int)(idx / N), k = (int)(idx % N);
+    int p, q;
+    if (e < N) { p = B; q = e; }
+    else { q = B; p = e - N; if (p >= B) ++p; }
+    G[((long)p * N + q) * N + k] = compact ? compact[idx] : grid[(long)N * N * k + (long)N * q + p];
+}
+
+// Inverse DWT: every wave owns its nodes outright (no reduction).  Workgroups
+// past the last pair copy the stale Nyquist entries (nq_grid or nq_compact
+// set): no pair writes those entries, and in the kernel's tail the copy costs
+// less than a kernel of its own.
 template <int NCH, int WPP, bool SMALL>
 __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) void k_dwt_inv_f64(PlanTables t, const double2* __restrict__ coeffs,
-                                                          double2* __restrict__ G, double norm)
+                                                          double2* __restrict__ G, double norm,
+                                                          const double2* __restrict__ nq_grid,
+                                                          const double2* __restrict__ nq_compact)
 {
+    if ((int)blockIdx.x >= t.npairs) {
+        nyquist_gather_one(nq_grid, G, t.B, nq_compact, (long)(blockIdx.x - t.npairs) * (64 * WPP) + threadIdx.x);
+        return;
+    }
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NCH);
     constexpr int WREG = inv_wave_region(NCH);  // a wave's tile, reused as its output staging
@@ -623,18 +650,23 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
 }
 
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st,
-                              int variant)
+                              int variant, const double2* nq_grid, const double2* nq_compact)
 {
     if (t.N > 1024) return hipErrorInvalidValue;
-    if (t.N > 256) return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
+    if (t.N > 256) {
+        if (nq_grid || nq_compact) return hipErrorInvalidValue;  // gathered by its own kernel there
+        return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
+    }
     const DwtShape s = dwt_shape(t.N, false, variant);
     const size_t smem = dwt_smem(t.B, t.N, false, variant);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-    dim3 grid(t.npairs), block(64 * s.wpp);
+    const long nq_total = (nq_grid || nq_compact) ? (long)(2 * t.N - 1) * t.N : 0;
+    const int nq_blocks = (int)((nq_total + 64 * s.wpp - 1) / (64 * s.wpp));
+    dim3 grid(t.npairs + nq_blocks), block(64 * s.wpp);
 #define PFS_INV(NC, W)                                                                                      \
     else if (s.nch == NC && s.wpp == W)                                                                     \
-        pfs::launch((k_dwt_inv_f64<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm);
-    if (t.N < 64) pfs::launch((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm);
+        pfs::launch((k_dwt_inv_f64<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm, nq_grid, nq_compact);
+    if (t.N < 64) pfs::launch((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm, nq_grid, nq_compact);
     PFS_INV(1, 1)
     PFS_INV(2, 1)
     PFS_INV(2, 2)
@@ -645,23 +677,12 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     return hipGetLastError();
 }
 
-// G[p][q][k] <- grid(row p, col q, slice k) for p == B or q == B.
-// The (2N - 1) N stale Nyquist entries -- spectral row p = B and column q = B
-// of every slice -- either read from the caller's grid or, when compact is
-// set, from a packed copy [e][k] (e < N: (p, q) = (B, e); e >= N: q = B,
-// p = e - N, skipping p = B) that a host caller uploads instead of the grid.
+// The stale Nyquist entries on their own (grids of 2B >= 512, whose inverse
+// DWT kernel does not carry them).
 __global__ void k_nyquist_gather_f64(const double2* __restrict__ grid, double2* __restrict__ G, int B,
                                      const double2* __restrict__ compact)
 {
-    const int N = 2 * B;
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = (long)(2 * N - 1) * N;
-    if (idx >= total) return;
-    int e = (int)(idx / N), k = (int)(idx % N);
-    int p, q;
-    if (e < N) { p = B; q = e; }
-    else { q = B; p = e - N; if (p >= B) ++p; }
-    G[((long)p * N + q) * N + k] = compact ? compact[idx] : grid[(long)N * N * k + (long)N * q + p];
+    nyquist_gather_one(grid, G, B, compact, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hipStream_t st, const double2* compact)

@@ -371,14 +371,14 @@ template <int Rn, int SIGN, typename C> __device__ __forceinline__ void dft_smal
 }
 
 // Lines per workgroup of the register kernel: the tile-side accesses move
-// whole runs of one column element over the workgroup's lines.  Measured on
-// MI355X at 2B = 256: complex128 tile writes prefer 8 lines (128-byte runs),
-// complex128 tile reads 16 (256 bytes); complex64 passes, 64 grids per launch,
-// run 4-9 % faster at 32 lines (256 bytes) than at 16.
+// whole runs of one column element over the workgroup's lines, 256 bytes at
+// 16 complex128 or 32 complex64 lines.  Measured on MI355X at 2B = 256: four
+// complex128 passes 350.1 us at 8 lines for the tile writes (128-byte runs),
+// 347.2 us at 16; complex64 passes, 64 grids per launch, 4-9 % faster at 32
+// lines than at 16.
 template <typename T, int KIND> constexpr int r16_lines()
 {
-    if (sizeof(T) == 4) return 32;
-    return KIND == LINES_ROW_IN_TILE_OUT ? 8 : 16;
+    return sizeof(T) == 4 ? 32 : 16;
 }
 // Tile stores of complex64 passes are non-temporal.  Those passes stream many
 // grids (8.6 GB for 64 grids at 2B = 256), far beyond the Infinity Cache, and

@@ -138,9 +138,12 @@ hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m,
 // pair); 0 is the default, see dwt.hip.
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm,
                               hipStream_t st, int variant = 0);
-// Inverse: packed coefficients -> G pencils (entries with p == B or q == B untouched).
+// Inverse: packed coefficients -> G pencils.  Entries with p == B or q == B
+// are untouched, or, for 2B <= 256 with nq_grid or nq_compact set, copied as
+// launch_nyquist_gather_f64 would by the same launch.
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm,
-                              hipStream_t st, int variant = 0);
+                              hipStream_t st, int variant = 0, const double2* nq_grid = nullptr,
+                              const double2* nq_compact = nullptr);
 
 // Variants for 2B >= 512 (dwt_large.hip).
 hipError_t launch_dwt_fwd_f64_large(const PlanTables& t, const double2* G, double2* coeffs, double norm,

@@ -188,6 +188,25 @@ def test_idsoft_reads_stale_nyquist_like_reference(ctx, B):
     assert rel(got, ref) < TOL
 
 
+@pytest.mark.parametrize("B,variant", [(4, 0), (32, 0), (64, 1), (128, 0), (128, 1)])
+def test_idsoft_dev_reads_stale_nyquist_like_reference(ctx, torch, B, variant):
+    # device buffers: the inverse DWT launch itself copies the stale entries
+    # (one to four waves per workgroup across these shapes)
+    N = 2 * B
+    c = O.seeded_coeffs(B, 6)
+    rng = np.random.default_rng(4)
+    stale = rng.standard_normal(N ** 3) + 1j * rng.standard_normal(N ** 3)
+    gd = torch.from_numpy(stale.copy()).cuda()
+    cd = torch.from_numpy(c).cuda()
+    ctx.tune("dwt_variant", variant)
+    try:
+        ctx.idsoft_dev(B, cd, gd, stream=torch.cuda.current_stream())
+        torch.cuda.synchronize()
+    finally:
+        ctx.tune("dwt_variant", 0)
+    assert rel(gd.cpu().numpy(), O.idsoft(B, c, grid=stale)) < TOL
+
+
 def test_dsoft_leaves_input_grid_untouched(ctx, torch):
     B = 16
     N = 2 * B
