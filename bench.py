@@ -7,11 +7,13 @@ B = 128 -- seeded coefficients f^l_{M,M'} (0 <= l < 128, uniform[-1,1]/(l+1),
 DSOFT -> coefficients.  One step = one iDSOFT + one DSOFT on device-resident
 buffers; value = round-trip pairs per second.
 
-Also reported: per-kernel average durations from HIP events recorded on the
-launch stream over the timed region, the roofline figure of the dominant
-kernel, per-step latency percentiles, the round-trip error, and the CPU
-oracle (C/OpenMP restatement of the reference) timed on this host on a
-bounded sample of the same workload.
+Also reported: the roofline figure of the dominant kernel, whose average
+duration comes from HIP events recorded in its launches' dispatch packets on
+the launch stream over the timed region (the only kernel timed there); a
+per-kernel breakdown from a separate pass with every kernel timed; per-step
+latency percentiles, the round-trip error, and the CPU oracle (C/OpenMP
+restatement of the reference) timed on this host on a bounded sample of the
+same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--bandwidth B]
 """
@@ -444,10 +446,33 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    # per-kernel times: start/stop events in every launch's own dispatch packet
-    # (no extra packets in the stream); per-step latency: one event per step
-    # boundary (step i runs from marks[i] to marks[i + 1])
+    def kernel_table(timings, steps):
+        table = {}
+        for name, (cnt, tot_ms) in timings.items():
+            per_step_s = tot_ms / steps / 1e3
+            ab = algorithmic_bytes_per_step(name, B)
+            table[name] = {"launches_per_step": cnt // steps, "us_per_step": round(per_step_s * 1e6, 2),
+                           "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "alg_bytes_per_step": ab,
+                           "achieved_GBps": round(ab / per_step_s / 1e9, 1),
+                           "frac_hbm": round(ab / per_step_s / 1e9 / HBM_PEAK_GBS, 4)}
+        return table
+
+    # Per-kernel times come from start/stop events in each timed launch's own
+    # dispatch packet.  Each timed launch adds ~3.5 us of device time, so the
+    # breakdown of every kernel comes from a pass of its own, outside the timed
+    # region, which then times only the kernel with the largest share.
+    nbreak = max(10, args.steps // 4)
     ctx.timing(True)
+    for _ in range(nbreak):
+        step()
+    torch.cuda.synchronize()
+    ctx.timing(False)
+    kernels = kernel_table(ctx.timing_collect(), nbreak)
+    dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
+
+    # timed region; per-step latency from one event per step boundary (step i
+    # runs from marks[i] to marks[i + 1])
+    ctx.timing(True, only=[dom])
     marks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -458,17 +483,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.timing(False)
-
-    kernels = {}
-    for name, (cnt, tot_ms) in ctx.timing_collect().items():
-        per_step_s = tot_ms / args.steps / 1e3
-        ab = algorithmic_bytes_per_step(name, B)
-        kernels[name] = {"launches_per_step": cnt // args.steps, "us_per_step": round(per_step_s * 1e6, 2),
-                         "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "alg_bytes_per_step": ab,
-                         "achieved_GBps": round(ab / per_step_s / 1e9, 1),
-                         "frac_hbm": round(ab / per_step_s / 1e9 / HBM_PEAK_GBS, 4)}
-    dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
-    dk = kernels[dom]
+    dk = kernel_table(ctx.timing_collect(), args.steps)[dom]
 
     lat = sorted(marks[i].elapsed_time(marks[i + 1]) for i in range(args.steps))
 
@@ -495,7 +510,8 @@ def main():
                    "bandwidth": B, "grid": [N, N, N], "precision": "fp64", "signals_per_step": 1},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": measured_traffic(B, dom),
-                     "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"])},
+                     "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"]),
+                     "avg_launch_us": dk["avg_launch_us"]},
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         # kernel time per direction (sum of its kernels' average durations)
         "idsoft_ms": {"kernels_avg": kernel_ms(["dwt_inv", "nyquist_gather", "fft_inv_cols", "fft_inv_rows"])},
@@ -503,6 +519,8 @@ def main():
         "transforms_per_s": 2 * args.steps / elapsed,
         "roundtrip_rel_l2": err,
         "kernels": kernels,
+        "kernels_note": "per-kernel breakdown from a separate pass of %d round trips with every kernel timed; "
+                        "the timed region times only %s (the roofline kernel)" % (nbreak, dom),
     }
     # PCIe-inclusive rate of the host-buffer entry points (not the headline value)
     c_host = coeffs.cpu().numpy()

@@ -89,6 +89,7 @@ def lib():
             "pfs_tune_get": (ip, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]),
             "pfs_timing_enable": (ip, [vp, ip]),
             "pfs_timing_collect": (ip, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+            "pfs_timing_select": (ip, [vp, ctypes.c_char_p]),
             "pfs_device_alloc": (ip, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
             "pfs_device_free": (ip, [vp, vp]),
             "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),
@@ -179,7 +180,9 @@ class Context:
     def tune(self, key, value):
         _check(lib().pfs_tune_set(self._h, key.encode(), int(value)))
 
-    def timing(self, on):
+    def timing(self, on, only=None):
+        """Per-kernel timing on or off; only: kernel names to time (None: all)."""
+        _check(lib().pfs_timing_select(self._h, ",".join(only).encode() if only else None))
         _check(lib().pfs_timing_enable(self._h, 1 if on else 0))
 
     def timing_collect(self):

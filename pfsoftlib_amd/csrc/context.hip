@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -257,6 +258,7 @@ struct pfs_context {
     // per-kernel timing: a start/stop event pair recorded around every launch
     // on the launch stream while enabled
     bool timing = false;
+    std::set<std::string> timing_only;  // kernels to time (empty: all)
     std::vector<hipEvent_t> ev_pool;
     struct Rec { const char* name; hipEvent_t a, b; };
     std::vector<Rec> recs;
@@ -302,7 +304,7 @@ hipEvent_t take_event(pfs_context* c)
 template <typename F>
 hipError_t timed(pfs_context* c, const char* name, hipStream_t st, F&& launch)
 {
-    if (!c->timing) return launch();
+    if (!c->timing || (!c->timing_only.empty() && !c->timing_only.count(name))) return launch();
     hipEvent_t a = take_event(c), b = take_event(c);
     LaunchTiming& t = pending_timing();
     t.start = a;
@@ -1396,6 +1398,25 @@ int pfs_timing_enable(pfs_context* c, int on)
     if (!c) return fail(PFS_ERR_ARG, "null context");
     std::lock_guard<std::mutex> g(c->mu);
     c->timing = on != 0;
+    return PFS_OK;
+}
+
+int pfs_timing_select(pfs_context* c, const char* names)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->timing_only.clear();
+    if (!names) return PFS_OK;
+    std::string cur;
+    for (const char* q = names;; ++q) {
+        if (*q == ',' || *q == 0) {
+            if (!cur.empty()) c->timing_only.insert(cur);
+            cur.clear();
+            if (*q == 0) break;
+        } else if (*q != ' ') {
+            cur += *q;
+        }
+    }
     return PFS_OK;
 }
 

@@ -303,3 +303,28 @@ def test_fft_chunking_is_bitwise_invariant(ctx, torch, B):
     ctx.tune("fft_chunk_slices", 0)
     for g, r in outs[1:]:
         assert np.array_equal(g, outs[0][0]) and np.array_equal(r, outs[0][1])
+
+
+def test_timing_select_times_only_the_named_kernels(ctx, torch):
+    B = 16
+    N = 2 * B
+    c = torch.from_numpy(O.seeded_coeffs(B, 9)).cuda()
+    g = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
+    back = torch.empty_like(c)
+    s = torch.cuda.current_stream()
+    try:
+        ctx.timing(True, only=["dwt_inv", "fft_fwd_cols"])
+        for _ in range(3):
+            ctx.idsoft_dev(B, c, g, stream=s)
+            ctx.dsoft_dev(B, g, back, stream=s)
+        ctx.timing(False)
+        t = ctx.timing_collect()
+        assert sorted(t) == ["dwt_inv", "fft_fwd_cols"] and all(v[0] == 3 and v[1] > 0 for v in t.values())
+        ctx.timing(True)  # no list: every kernel again
+        ctx.idsoft_dev(B, c, g, stream=s)
+        ctx.dsoft_dev(B, g, back, stream=s)
+        ctx.timing(False)
+        assert {"dwt_inv", "dwt_fwd", "fft_inv_cols", "fft_inv_rows", "fft_fwd_rows", "fft_fwd_cols"} <= set(ctx.timing_collect())
+    finally:
+        ctx.timing(False)
+        ctx.timing_collect()
