@@ -186,38 +186,6 @@ __device__ __forceinline__ double2 load_nt(const double2* p)
     return r;
 }
 
-// Lane rr of each 16-lane row of v, in every lane of that row: one DPP move
-// (row_newbcast:rr) into a VGPR.  Two v_readlane into SGPRs cost twice the
-// VALU issue, and a VALU op may read only one SGPR pair, so the SGPR form
-// also needed a copy back to a VGPR per step (measured: 116.8 -> 112.6 us
-// forward, 125.7 -> 121.8 us inverse at B = 128).
-template <int RR> __device__ __forceinline__ double row_bcast_c(double v)
-{
-    return __builtin_amdgcn_mov_dpp(v, 0x150 + RR, 0xf, 0xf, false);
-}
-// rr is a constant once the caller's loop is unrolled; the switch folds away.
-__device__ __forceinline__ double row_bcast(double v, int rr)
-{
-    switch (rr) {
-    case 0: return row_bcast_c<0>(v);
-    case 1: return row_bcast_c<1>(v);
-    case 2: return row_bcast_c<2>(v);
-    case 3: return row_bcast_c<3>(v);
-    case 4: return row_bcast_c<4>(v);
-    case 5: return row_bcast_c<5>(v);
-    case 6: return row_bcast_c<6>(v);
-    case 7: return row_bcast_c<7>(v);
-    case 8: return row_bcast_c<8>(v);
-    case 9: return row_bcast_c<9>(v);
-    case 10: return row_bcast_c<10>(v);
-    case 11: return row_bcast_c<11>(v);
-    case 12: return row_bcast_c<12>(v);
-    case 13: return row_bcast_c<13>(v);
-    case 14: return row_bcast_c<14>(v);
-    default: return row_bcast_c<15>(v);
-    }
-}
-
 // Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
 // column 64 (c - C0) + lane) and advances them.  Lane L fetches the step
 // coefficients of degree r0 + L % 16 in one LDS read each, so every 16-lane

@@ -132,16 +132,19 @@ __device__ __forceinline__ void init_chains(Chains<NCH>& ch, const PlanTables& t
 }
 
 // Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
-// column 64 (c - C0) + lane) and advances them.  The step coefficients are
-// read once per degree for these chains.
+// column 64 (c - C0) + lane) and advances them.  Lane L reads the step
+// coefficients of degree r0 + L % 16 once; each step broadcasts its own
+// within the 16-lane rows.  Measured at B = 256 against three LDS broadcast
+// reads per step: forward 1849 -> 1740 us, inverse 1604 -> 1480 us.
 template <int NCH, int C0, int NC>
 __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* tile, int tstr, int lane, const double* cf,
                                            int RP, int r0)
 {
+    const int rl = r0 + (lane & 15);
+    const double vf1 = cf[rl], vf1f2 = cf[RP + rl], vc1 = cf[2 * RP + rl];
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
-        const int r = r0 + rr;
-        const double f1 = cf[r], f1f2 = cf[RP + r], c1 = cf[2 * RP + r];
+        const double f1 = row_bcast(vf1, rr), f1f2 = row_bcast(vf1f2, rr), c1 = row_bcast(vc1, rr);
 #pragma unroll
         for (int c = C0; c < C0 + NC; ++c) {
             tile[rr * tstr + 64 * (c - C0) + lane] = ch.cur[c];
