@@ -501,9 +501,8 @@ hipError_t fwd_launch(const PlanTables& t, const BatchTables& bt, const float2* 
         return hipErrorInvalidValue;
     } else {
         const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
-        pfs::launch((k_dwt_fwd_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), 0, st, t,
+        return pfs::launch((k_dwt_fwd_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), 0, st, t,
                            bt.roff, bt.wts32, bt.tf, G, coeffs, nsig, nsg, cdist, cstride, norm, g_mfma_f32_layout);
-        return hipGetLastError();
     }
 }
 
@@ -517,9 +516,8 @@ hipError_t inv_launch(const PlanTables& t, const BatchTables& bt, const float2* 
         const size_t smem = sizeof(float) * (size_t)WPB * NN * OSTRF;
         if (smem > lds_limit()) return hipErrorInvalidConfiguration;
         const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
-        pfs::launch((k_dwt_inv_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), smem, st, t,
+        return pfs::launch((k_dwt_inv_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), smem, st, t,
                            bt.roff, bt.ti, coeffs, G, nsig, nsg, cdist, cstride, norm, g_mfma_f32_layout);
-        return hipGetLastError();
     }
 }
 
@@ -558,9 +556,8 @@ bool batch_supported(int N) { return N == 16 || N == 32 || N == 64 || N == 128 |
 
 hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st)
 {
-    pfs::launch(k_wigner_tiles_f32, dim3(t.npairs), dim3(t.N < 256 ? t.N : 256), 0, st, t, bt.roff,
+    return pfs::launch(k_wigner_tiles_f32, dim3(t.npairs), dim3(t.N < 256 ? t.N : 256), 0, st, t, bt.roff,
                        (float*)bt.wts32, (float*)bt.tf, (float*)bt.ti);
-    return hipGetLastError();
 }
 
 BatchKernelShape batch_default_shape(int N, bool fwd, int n)
@@ -615,19 +612,15 @@ hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float
     if (n < 1 || ld < n) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((count + 63) / 64), (unsigned)((n + 31) / 32));
     if (to_interleaved)
-        pfs::launch(k_coeff_transpose<true>, grid, dim3(256), 0, st, per_signal, sptr, interleaved, count, n, ld);
-    else
-        pfs::launch(k_coeff_transpose<false>, grid, dim3(256), 0, st, per_signal, sptr, interleaved, count, n,
-                           ld);
-    return hipGetLastError();
+        return pfs::launch(k_coeff_transpose<true>, grid, dim3(256), 0, st, per_signal, sptr, interleaved, count, n, ld);
+    return pfs::launch(k_coeff_transpose<false>, grid, dim3(256), 0, st, per_signal, sptr, interleaved, count, n, ld);
 }
 
 hipError_t launch_nyquist_zero_f32(float2* G, int B, int nsig, hipStream_t st)
 {
     const int N = 2 * B;
     const long total = (long)(2 * N - 1) * N * nsig;
-    pfs::launch(k_nyquist_zero_f32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, G, B, nsig);
-    return hipGetLastError();
+    return pfs::launch(k_nyquist_zero_f32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, G, B, nsig);
 }
 
 }  // namespace pfs

@@ -162,9 +162,8 @@ int mfma_selftest_f32(hipStream_t st)
 {
     float* d = nullptr;
     HIP_TRY(hipMalloc(&d, 512 * sizeof(float)));
-    pfs::launch(k_mfma_probe_f32, dim3(1), dim3(64), 0, st, d);
+    hipError_t e = pfs::launch(k_mfma_probe_f32, dim3(1), dim3(64), 0, st, d);
     float h[512];
-    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(d);
@@ -194,9 +193,8 @@ int mfma_selftest(hipStream_t st)
 {
     double* d = nullptr;
     HIP_TRY(hipMalloc(&d, 512 * sizeof(double)));
-    pfs::launch(k_mfma_probe, dim3(1), dim3(64), 0, st, d);
+    hipError_t e = pfs::launch(k_mfma_probe, dim3(1), dim3(64), 0, st, d);
     double h[512];
-    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(d);
@@ -400,13 +398,10 @@ int get_plan(pfs_context* c, int B, Plan** out)
     p.t.tw32 = (float2*)(base + off_tw32);
     hipError_t e = hipMemcpyAsync((void*)p.t.pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   c->stream);
-    if (e == hipSuccess) {
-        pfs::launch(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
-                           (double*)p.t.sinh, (double*)p.t.cosh, (double*)p.t.wts, (double2*)p.t.tw,
-                           (float2*)p.t.tw32);
-        pfs::launch(k_plan_normfac, dim3(B), dim3(64), 0, c->stream, B, (double*)p.t.normfac);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess)
+        e = pfs::launch(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
+                        (double*)p.t.sinh, (double*)p.t.cosh, (double*)p.t.wts, (double2*)p.t.tw, (float2*)p.t.tw32);
+    if (e == hipSuccess) e = pfs::launch(k_plan_normfac, dim3(B), dim3(64), 0, c->stream, B, (double*)p.t.normfac);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         ctx_free(c, p.mem, p.bytes);
@@ -1220,9 +1215,8 @@ int pfs_wigner_d_matrix(pfs_context* c, int B, int M, int Mp, int weighted, doub
     size_t bytes = (size_t)(B - M) * 2 * B * sizeof(double);
     rc = ensure_io(c, bytes, 0);
     if (rc) return rc;
-    pfs::launch(k_wigner_table, dim3((2 * B + 63) / 64), dim3(64), 0, c->stream, B, M, Mp, p->t.cosb,
-                       p->t.sinh, p->t.cosh, p->t.wts, p->t.normfac, weighted, (double*)c->io_grid);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(pfs::launch(k_wigner_table, dim3((2 * B + 63) / 64), dim3(64), 0, c->stream, B, M, Mp, p->t.cosb,
+                        p->t.sinh, p->t.cosh, p->t.wts, p->t.normfac, weighted, (double*)c->io_grid));
     HIP_TRY(hipMemcpyAsync(out, c->io_grid, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return PFS_OK;

@@ -603,8 +603,8 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
     dim3 grid(t.npairs), block(64 * s.wpp);
 #define PFS_FWD(NC, NH_, W)                                                                                 \
     else if (s.nch == NC && s.nh == NH_ && s.wpp == W)                                                      \
-        pfs::launch((k_dwt_fwd_f64<NC, NH_, W, false>), grid, block, smem, st, t, G, coeffs, norm);
-    if (t.N < 64) pfs::launch((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
+        return pfs::launch((k_dwt_fwd_f64<NC, NH_, W, false>), grid, block, smem, st, t, G, coeffs, norm);
+    if (t.N < 64) return pfs::launch((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
     PFS_FWD(1, 1, 1)
     PFS_FWD(2, 2, 1)
     PFS_FWD(4, 2, 1)
@@ -612,9 +612,8 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
     PFS_FWD(2, 2, 2)
     PFS_FWD(2, 1, 2)
     PFS_FWD(1, 1, 4)
-    else return hipErrorInvalidValue;
 #undef PFS_FWD
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st,
@@ -633,16 +632,16 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     dim3 grid(t.npairs + nq_blocks), block(64 * s.wpp);
 #define PFS_INV(NC, W)                                                                                      \
     else if (s.nch == NC && s.wpp == W)                                                                     \
-        pfs::launch((k_dwt_inv_f64<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm, nq_grid, nq_compact);
-    if (t.N < 64) pfs::launch((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm, nq_grid, nq_compact);
+        return pfs::launch((k_dwt_inv_f64<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm, nq_grid, nq_compact);
+    if (t.N < 64)
+        return pfs::launch((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm, nq_grid, nq_compact);
     PFS_INV(1, 1)
     PFS_INV(2, 1)
     PFS_INV(2, 2)
     PFS_INV(1, 2)
     PFS_INV(1, 4)
-    else return hipErrorInvalidValue;
 #undef PFS_INV
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 // The stale Nyquist entries on their own (grids of 2B >= 512, whose inverse
@@ -657,8 +656,7 @@ hipError_t launch_nyquist_gather_f64(const double2* grid, double2* G, int B, hip
 {
     const int N = 2 * B;
     long total = (long)(2 * N - 1) * N;
-    pfs::launch(k_nyquist_gather_f64, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, grid, G, B, compact);
-    return hipGetLastError();
+    return pfs::launch(k_nyquist_gather_f64, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, grid, G, B, compact);
 }
 
 }  // namespace pfs

@@ -479,13 +479,12 @@ hipError_t launch_fwd(const PlanTables& t, const double2* G, double2* coeffs, do
     size_t smem = dwt_smem(t.B, t.N, true);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     dim3 grid(t.npairs), block(64 * s.wpp);
-    if (t.N < 64) pfs::launch((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.nch == 1) pfs::launch((k_dwt_fwd_f64<1, 1, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.nch == 2) pfs::launch((k_dwt_fwd_f64<2, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.wpp == 1) pfs::launch((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else if (s.wpp == 2) pfs::launch((k_dwt_fwd_f64<4, 2, 2, false>), grid, block, smem, st, t, G, coeffs, norm);
-    else pfs::launch((k_dwt_fwd_f64<4, 2, 4, false>), grid, block, smem, st, t, G, coeffs, norm);
-    return hipGetLastError();
+    if (t.N < 64) return pfs::launch((k_dwt_fwd_f64<1, 1, 1, true>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.nch == 1) return pfs::launch((k_dwt_fwd_f64<1, 1, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.nch == 2) return pfs::launch((k_dwt_fwd_f64<2, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.wpp == 1) return pfs::launch((k_dwt_fwd_f64<4, 2, 1, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else if (s.wpp == 2) return pfs::launch((k_dwt_fwd_f64<4, 2, 2, false>), grid, block, smem, st, t, G, coeffs, norm);
+    else return pfs::launch((k_dwt_fwd_f64<4, 2, 4, false>), grid, block, smem, st, t, G, coeffs, norm);
 }
 
 hipError_t launch_inv(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st)
@@ -495,13 +494,12 @@ hipError_t launch_inv(const PlanTables& t, const double2* coeffs, double2* G, do
     size_t smem = dwt_smem(t.B, t.N, false);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     dim3 grid(t.npairs), block(64 * s.wpp);
-    if (t.N < 64) pfs::launch((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.nch == 1) pfs::launch((k_dwt_inv_f64<1, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.wpp == 1) pfs::launch((k_dwt_inv_f64<2, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.wpp == 2) pfs::launch((k_dwt_inv_f64<2, 2, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else if (s.wpp == 4) pfs::launch((k_dwt_inv_f64<2, 4, false>), grid, block, smem, st, t, coeffs, G, norm);
-    else pfs::launch((k_dwt_inv_f64<2, 8, false>), grid, block, smem, st, t, coeffs, G, norm);
-    return hipGetLastError();
+    if (t.N < 64) return pfs::launch((k_dwt_inv_f64<1, 1, true>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.nch == 1) return pfs::launch((k_dwt_inv_f64<1, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.wpp == 1) return pfs::launch((k_dwt_inv_f64<2, 1, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.wpp == 2) return pfs::launch((k_dwt_inv_f64<2, 2, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else if (s.wpp == 4) return pfs::launch((k_dwt_inv_f64<2, 4, false>), grid, block, smem, st, t, coeffs, G, norm);
+    else return pfs::launch((k_dwt_inv_f64<2, 8, false>), grid, block, smem, st, t, coeffs, G, norm);
 }
 
 }  // namespace lg

@@ -609,8 +609,7 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
         dim3 grid(m.nv / L, m.nu, nsig);
         const size_t smem = r16_lds<T, KIND>(LOGN);
         if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-        pfs::launch((k_fft_lines_r16x2<T, KIND, SIGN>), grid, dim3(L * 32), smem, st, in, out, m, tw, scale);
-        return hipGetLastError();
+        return pfs::launch((k_fft_lines_r16x2<T, KIND, SIGN>), grid, dim3(L * 32), smem, st, in, out, m, tw, scale);
     }
     if (LOGN >= 6 && LOGN <= 8) {
         constexpr int L = r16_lines<T, KIND>();
@@ -618,17 +617,15 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
         dim3 grid(m.nv / L, m.nu, nsig);
         const size_t smem = r16_lds<T, KIND>(LOGN);
         if (smem > lds_limit()) return hipErrorInvalidConfiguration;
-        pfs::launch((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
+        return pfs::launch((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
                            dim3(L * N / 16), smem, st, in, out, m, tw, scale);
-        return hipGetLastError();
     }
     constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;
     size_t smem = (size_t)LINES * (N + 1) * sizeof(typename CT<T>::C);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     if (m.nv % LINES != 0) return hipErrorInvalidValue;
     dim3 grid(m.nv / LINES, m.nu, nsig);
-    pfs::launch((k_fft_lines<T, LOGN, LINES, KIND, SIGN>), grid, dim3(256), smem, st, in, out, m, tw, scale);
-    return hipGetLastError();
+    return pfs::launch((k_fft_lines<T, LOGN, LINES, KIND, SIGN>), grid, dim3(256), smem, st, in, out, m, tw, scale);
 }
 
 template <typename T, int KIND, int SIGN>
@@ -676,18 +673,15 @@ static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* o
     size_t smem = 2 * (size_t)lines * (N + 1) * sizeof(typename CT<T>::C);
     if (m.nv % lines != 0) return hipErrorInvalidValue;
     dim3 grid(m.nv / lines, m.nu, nsig);
-    if (kind == LINES_ROW_IN_TILE_OUT) {
-        if (sign < 0)
-            pfs::launch((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
-        else
-            pfs::launch((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
-    } else {
-        if (sign < 0)
-            pfs::launch((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
-        else
-            pfs::launch((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw, scale, N, lines);
-    }
-    return hipGetLastError();
+    if (kind == LINES_ROW_IN_TILE_OUT)
+        return sign < 0 ? pfs::launch((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, -1>), grid, dim3(256), smem, st, in, out, m,
+                                      tw, scale, N, lines)
+                        : pfs::launch((k_dft_lines<T, LINES_ROW_IN_TILE_OUT, 1>), grid, dim3(256), smem, st, in, out, m,
+                                      tw, scale, N, lines);
+    return sign < 0 ? pfs::launch((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, -1>), grid, dim3(256), smem, st, in, out, m, tw,
+                                  scale, N, lines)
+                    : pfs::launch((k_dft_lines<T, LINES_TILE_IN_ROW_OUT, 1>), grid, dim3(256), smem, st, in, out, m, tw,
+                                  scale, N, lines);
 }
 
 size_t lds_limit()
