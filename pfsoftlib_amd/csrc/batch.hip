@@ -332,26 +332,38 @@ __device__ __forceinline__ void inv_gather(InvStage<SPW>& S, const float* const 
 }
 
 template <int NKB, int SPW>
-__device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStage<SPW>& S, const float* tq)
+__device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStage<SPW>& S, float4 (&a)[NKB],
+                                             const float* tnext)
 {
-    float4 a[NKB];
-#pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) a[kb] = *reinterpret_cast<const float4*>(tq + kb * 256);
     float b[SPW][4];
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp)
 #pragma unroll
         for (int u = 0; u < 4; ++u) b[sp][u] = S.v[sp][u] * S.f[u];
     __builtin_amdgcn_sched_barrier(0);
+    // Tiles in groups of four: a group's GS x SPW independent accumulator
+    // chains hide the MFMA latency, and its registers take the next chunk's
+    // tiles as soon as its MFMAs have read them, a chunk's worth of MFMAs
+    // before they are used (measured: 3.92 -> 3.61 ms for 64 signals at
+    // B = 128 against loading each chunk's tiles right before its MFMAs).
+    constexpr int GS = NKB < 4 ? NKB : 4;
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
+    for (int g = 0; g < NKB / GS; ++g) {
 #pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) {
-            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].x, b[sp][0], acc[kb][sp], 0, 0, 0);
-            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].y, b[sp][1], acc[kb][sp], 0, 0, 0);
-            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].z, b[sp][2], acc[kb][sp], 0, 0, 0);
-            acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb].w, b[sp][3], acc[kb][sp], 0, 0, 0);
-        }
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k4 = 0; k4 < GS; ++k4)
+#pragma unroll
+                for (int sp = 0; sp < SPW; ++sp) {
+                    const int kb = GS * g + k4;
+                    const float av = u == 0 ? a[kb].x : u == 1 ? a[kb].y : u == 2 ? a[kb].z : a[kb].w;
+                    acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[sp][u], acc[kb][sp], 0, 0, 0);
+                }
+#pragma unroll
+        for (int k4 = 0; k4 < GS; ++k4)
+            a[GS * g + k4] = *reinterpret_cast<const float4*>(tnext + (GS * g + k4) * 256);
+        __builtin_amdgcn_sched_group_barrier(0x008, 4 * GS * SPW, 0);  // the group's MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x020, GS, 0);            // then its next-chunk tile loads
     }
 }
 
@@ -397,11 +409,15 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
     // past R and for inactive cases, scaled by 0): a branch around them would
     // make the compiler drain all outstanding loads.
     InvStage<SPW> bc, bn;
+    float4 a[NKB];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) a[kb] = *reinterpret_cast<const float4*>(ti + kb * 256);
     inv_gather<SPW>(bc, cs, cb, M, R, cstride, norm, kk, 0);
     for (int q = 0; q < nch; ++q) {
-        inv_gather<SPW>(bn, cs, cb, M, R, cstride, norm, kk, q + 1 < nch ? q + 1 : q);
+        const int qn = q + 1 < nch ? q + 1 : q;
+        inv_gather<SPW>(bn, cs, cb, M, R, cstride, norm, kk, qn);
         __builtin_amdgcn_sched_barrier(0);
-        inv_contract<NKB, SPW>(acc, bc, ti + q * NKB * 256);
+        inv_contract<NKB, SPW>(acc, bc, a, ti + qn * NKB * 256);
         __builtin_amdgcn_sched_barrier(0);
         bc = bn;
     }
