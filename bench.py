@@ -175,10 +175,20 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
             k["useful_TFLOPs"] = round(tf, 2)
             k["frac_fp32_mfma"] = round(tf / FP32_MFMA_PEAK_TFS, 4)
         if name.startswith("batch_fft_"):
+            # a pass reads and writes every signal's grid once: its own traffic
             gbs = nsig * N ** 3 * 8 * 2 / (tot_ms / steps / 1e3) / 1e9
-            k["achieved_GBps"] = round(gbs, 1)
-            k["frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
+            k["pass_traffic_GBps"] = round(gbs, 1)
+            k["pass_traffic_frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
         kernels[name] = k
+    # FFT stage per direction, algorithmic: one read and one write of the grids
+    # (2 N^3 e per signal) over the time of both passes of that direction
+    fft_stage = {}
+    for d, names in (("forward", ("batch_fft_fwd_rows", "batch_fft_fwd_cols")),
+                     ("inverse", ("batch_fft_inv_cols", "batch_fft_inv_rows"))):
+        if all(x in kernels for x in names):
+            us = sum(kernels[x]["us_per_step"] for x in names)
+            gbs = nsig * 2 * N ** 3 * 8 / (us * 1e-6) / 1e9
+            fft_stage[d] = {"us": round(us, 2), "alg_GBps": round(gbs, 1), "alg_frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
 
     # error of every signal against the FP64 path on the same input
     g64 = torch.empty(N ** 3, dtype=torch.complex128, device="cuda")
@@ -204,6 +214,7 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
                            "inverse_median": sorted(e_inv)[nchk // 2], "forward_median": sorted(e_fwd)[nchk // 2],
                            "signals_checked": nchk, "bound": 1e-6},
         "kernels": kernels,
+        "fft_stage": fft_stage,
     }
 
 
@@ -393,6 +404,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--fft-chunk", type=int, default=0, help="beta-slices per FFT chunk (0 = automatic)")
     ap.add_argument("--dwt-variant", type=int, default=0, help="work shape of the FP64 DWT kernels (0 = default)")
+    ap.add_argument("--dwt-pair-order", type=int, default=-1,
+                    help="pair launch order of the FP64 DWT kernels, bit 0 forward, bit 1 inverse (0 = per-XCD M' "
+                         "ranges, 1 = runs of 8 M' per XCD; -1 = library default)")
     ap.add_argument("--batch-signals", type=int, default=64,
                     help="signals per batched mixed-precision round trip at the same bandwidth (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -428,6 +442,8 @@ def main():
     ctx.warm(B)
     ctx.tune("fft_chunk_slices", args.fft_chunk)
     ctx.tune("dwt_variant", args.dwt_variant)
+    if args.dwt_pair_order >= 0:
+        ctx.tune("dwt_pair_order", args.dwt_pair_order)
     if args.batch_fft_signals >= 0:
         ctx.tune("batch_fft_signals", args.batch_fft_signals)
     if args.batch_fft_slices >= 0:
@@ -490,6 +506,29 @@ def main():
     def kernel_ms(names):
         return round(sum(kernels[k]["us_per_step"] for k in names if k in kernels) / 1e3, 4)
 
+    def fft_stage():
+        """Slice-FFT stage per direction: algorithmic bytes (one read and one
+        write of the grid, 2 N^3 e) over the time of its two passes, and per
+        pass the HBM traffic the committed FETCH_SIZE/WRITE_SIZE counters give
+        for that pass over its time (the row/column split moves the grid through
+        HBM once per pass)."""
+        res = {}
+        for d, names in (("forward", ("fft_fwd_rows", "fft_fwd_cols")), ("inverse", ("fft_inv_cols", "fft_inv_rows"))):
+            if not all(n in kernels for n in names):
+                continue
+            us = sum(kernels[n]["us_per_step"] for n in names)
+            alg = 2 * N ** 3 * 16
+            passes = {}
+            for n in names:
+                tr = measured_traffic(B, n)
+                if tr:
+                    gbs = tr / (kernels[n]["avg_launch_us"] * 1e-6) / 1e9
+                    passes[n] = {"traffic_bytes": tr, "traffic_GBps": round(gbs, 1),
+                                 "traffic_frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+            res[d] = {"us": round(us, 2), "alg_bytes": alg, "alg_GBps": round(alg / (us * 1e-6) / 1e9, 1),
+                      "alg_frac_hbm": round(alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "passes": passes}
+        return res
+
     err = (torch.linalg.vector_norm(back - coeffs) / torch.linalg.vector_norm(coeffs)).item()
     elapsed = t1 - t0
     out = {
@@ -519,6 +558,7 @@ def main():
         "transforms_per_s": 2 * args.steps / elapsed,
         "roundtrip_rel_l2": err,
         "kernels": kernels,
+        "fft_stage": fft_stage(),
         "kernels_note": "per-kernel breakdown from a separate pass of %d round trips with every kernel timed; "
                         "the timed region times only %s (the roofline kernel)" % (nbreak, dom),
     }

@@ -228,6 +228,10 @@ struct Plan {
     // batched contraction shapes picked by the autotuner per direction and
     // batch-size bucket (ceil(log2 n), capped at 7); {0, 0} = default rule
     BatchKernelShape tuned[2][8] = {};
+    // launch orders of the pairs for the FP64 DWT kernels: 0 = per-XCD M'
+    // ranges (xcd_pair_order), 1 = runs of 8 consecutive M' per XCD within
+    // windows of 64 pairs
+    const int2* xorder[2] = {nullptr, nullptr};
 };
 
 }  // namespace
@@ -265,6 +269,11 @@ struct pfs_context {
     long batch_chunk_signals = 64;
     long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
     long dwt_variant = 0;       // work shape of the FP64 DWT kernels (2B <= 256)
+    // pair launch order of the FP64 DWT kernels (Plan::xorder): bit 0 the
+    // forward's, bit 1 the inverse's.  Measured at B = 128: the per-XCD M'
+    // ranges halve the inverse's HBM fetch (183 -> 96 MB) and cut the forward's
+    // writes (71 -> 54 MB) at the same kernel times (within 1 us), so 0 (both).
+    long dwt_pair_order = 0;
     long server_graphs = 1;     // request servers replay captured HIP graphs
     long server_autotune = 1;   // request servers autotune the batched shapes at creation
     long batch_force_shape = 0; // 0: tuned / default shapes; else 10 * waves + signals per wave
@@ -352,6 +361,40 @@ int check_bandwidth(int B)
     return PFS_OK;
 }
 
+// Launch order of the canonical pairs for the FP64 DWT kernels (one pair per
+// workgroup; workgroup b runs on XCD b % 8, the round-robin dispatch of the
+// device).  Pairs stay heaviest first (M ascending).  The pairs M' = 0..M of
+// each M are split into 8 contiguous M' ranges and range x goes to XCD x, so
+// one XCD works at a time on neighbouring M' of one M and on the same M'
+// range of the next few M.  In the packed column-major coefficient blocks,
+// (l, +-M', .) are neighbours for consecutive M' and (l, +-M, .) for
+// consecutive M: one L2 then serves the coefficient lines of all eight cases
+// of neighbouring pairs.  Which XCD gets a range's extra pair rotates with M,
+// so the eight queues stay within one pair of each other.
+std::vector<int2> xcd_pair_order(int B)
+{
+    constexpr int X = 8;
+    std::vector<std::vector<int2>> q(X);
+    int rot = 0;
+    for (int M = 0; M < B; ++M) {
+        const int n = M + 1, base = n / X, rem = n % X;
+        int Mp = 0;
+        for (int x = 0; x < X; ++x) {
+            const int sz = base + (((x - rot + X) % X) < rem ? 1 : 0);
+            for (int i = 0; i < sz; ++i, ++Mp) q[x].push_back(make_int2(M, Mp));
+        }
+        rot = (rot + rem) % X;
+    }
+    std::vector<int2> order;
+    order.reserve((size_t)B * (B + 1) / 2);
+    size_t maxlen = 0;
+    for (auto& v : q) maxlen = std::max(maxlen, v.size());
+    for (size_t j = 0; j < maxlen; ++j)
+        for (int x = 0; x < X; ++x)
+            if (j < q[x].size()) order.push_back(q[x][j]);
+    return order;
+}
+
 int get_plan(pfs_context* c, int B, Plan** out)
 {
     int rc = check_bandwidth(B);
@@ -375,7 +418,15 @@ int get_plan(pfs_context* c, int B, Plan** out)
     size_t off_wts = off_cosh + align(N * sizeof(double));
     size_t off_nf = off_wts + align(N * sizeof(double));
     size_t off_pairs = off_nf + align((size_t)B * B * sizeof(double));
-    size_t off_tw = off_pairs + align(pairs.size() * sizeof(int2));
+    const std::vector<int2> xpairs = xcd_pair_order(B);
+    std::vector<int2> wpairs(pairs.size());
+    for (size_t b = 0; b < pairs.size(); ++b) {
+        const size_t g = b >> 6, r = b & 63;
+        wpairs[b] = (g + 1) * 64 > pairs.size() ? pairs[b] : pairs[g * 64 + (r & 7) * 8 + (r >> 3)];
+    }
+    size_t off_xpairs = off_pairs + align(pairs.size() * sizeof(int2));
+    size_t off_wpairs = off_xpairs + align(xpairs.size() * sizeof(int2));
+    size_t off_tw = off_wpairs + align(wpairs.size() * sizeof(int2));
     size_t off_tw32 = off_tw + align(N * sizeof(double2));
     size_t total = off_tw32 + align(N * sizeof(float2));
 
@@ -394,10 +445,19 @@ int get_plan(pfs_context* c, int B, Plan** out)
     p.t.wts = (double*)(base + off_wts);
     p.t.normfac = (double*)(base + off_nf);
     p.t.pairs = (int2*)(base + off_pairs);
+    p.t.xpairs = (int2*)(base + off_xpairs);
+    p.xorder[0] = p.t.xpairs;
+    p.xorder[1] = (int2*)(base + off_wpairs);
     p.t.tw = (double2*)(base + off_tw);
     p.t.tw32 = (float2*)(base + off_tw32);
     hipError_t e = hipMemcpyAsync((void*)p.t.pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((void*)p.t.xpairs, xpairs.data(), xpairs.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((void*)p.xorder[1], wpairs.data(), wpairs.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           c->stream);
     if (e == hipSuccess)
         e = pfs::launch(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
                         (double*)p.t.sinh, (double*)p.t.cosh, (double*)p.t.wts, (double2*)p.t.tw, (float2*)p.t.tw32);
@@ -637,6 +697,15 @@ int fft_inv(pfs_context* c, const Plan& p, const double2* G, double2* W, double2
     return PFS_OK;
 }
 
+// The plan's tables with the FP64 DWT launch order the context selects for
+// one direction.
+PlanTables dwt_tables(const pfs_context* c, const Plan& p, bool fwd)
+{
+    PlanTables t = p.t;
+    t.xpairs = p.xorder[(c->dwt_pair_order >> (fwd ? 0 : 1)) & 1];
+    return t;
+}
+
 // pi / (B * 2B) and (B * 2B) / pi in double (src/fn_dsoft.cpp:119, src/fn_idsoft.cpp:109)
 inline double fwd_norm(int B) { return 3.14159265358979323846 / (B * (2 * B)); }
 inline double inv_norm(int B) { return (B * (2 * B)) / 3.14159265358979323846; }
@@ -652,7 +721,7 @@ int dsoft_dev(pfs_context* c, int B, const double2* grid, double2* coeffs, hipSt
     double2* G = (double2*)c->work[1];
     rc = fft_fwd(c, *p, grid, W, G, st);
     if (rc) return rc;
-    HIP_TRY(timed(c, "dwt_fwd", st, [&] { return launch_dwt_fwd_f64(p->t, G, coeffs, fwd_norm(B), st, (int)c->dwt_variant); }));
+    HIP_TRY(timed(c, "dwt_fwd", st, [&] { return launch_dwt_fwd_f64(dwt_tables(c, *p, true), G, coeffs, fwd_norm(B), st, (int)c->dwt_variant); }));
     return PFS_OK;
 }
 
@@ -672,7 +741,7 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     const bool fused = p->t.N <= 256;
     if (!fused) HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st, stale); }));
     HIP_TRY(timed(c, "dwt_inv", st, [&] {
-        return launch_dwt_inv_f64(p->t, coeffs, G, inv_norm(B), st, (int)c->dwt_variant, fused && !stale ? grid : nullptr,
+        return launch_dwt_inv_f64(dwt_tables(c, *p, false), coeffs, G, inv_norm(B), st, (int)c->dwt_variant, fused && !stale ? grid : nullptr,
                                   fused ? stale : nullptr);
     }));
     return fft_inv(c, *p, G, W, grid, st);
@@ -1243,7 +1312,7 @@ int pfs_stage_dwt_fwd_dev(pfs_context* c, int B, const void* pencils_dev, void* 
     Plan* p;
     int rc = get_plan(c, B, &p);
     if (rc) return rc;
-    HIP_TRY(launch_dwt_fwd_f64(p->t, (const double2*)pencils_dev, (double2*)coeffs_dev, fwd_norm(B), pick(c, stream),
+    HIP_TRY(launch_dwt_fwd_f64(dwt_tables(c, *p, true), (const double2*)pencils_dev, (double2*)coeffs_dev, fwd_norm(B), pick(c, stream),
                                (int)c->dwt_variant));
     return PFS_OK;
 }
@@ -1256,7 +1325,7 @@ int pfs_stage_dwt_inv_dev(pfs_context* c, int B, const void* coeffs_dev, void* p
     Plan* p;
     int rc = get_plan(c, B, &p);
     if (rc) return rc;
-    HIP_TRY(launch_dwt_inv_f64(p->t, (const double2*)coeffs_dev, (double2*)pencils_dev, inv_norm(B), pick(c, stream),
+    HIP_TRY(launch_dwt_inv_f64(dwt_tables(c, *p, false), (const double2*)coeffs_dev, (double2*)pencils_dev, inv_norm(B), pick(c, stream),
                                (int)c->dwt_variant));
     return PFS_OK;
 }
@@ -1309,6 +1378,11 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->batch_interleave = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "dwt_pair_order")) {
+        if (value < 0 || value > 3) return fail(PFS_ERR_ARG, "dwt_pair_order must be in 0..3 (bit 0 forward, bit 1 inverse)");
+        c->dwt_pair_order = value;
+        return PFS_OK;
+    }
     if (!strcmp(key, "dwt_variant")) {
         if (value < 0 || value > 3) return fail(PFS_ERR_ARG, "dwt_variant must be in 0..3");
         c->dwt_variant = value;
@@ -1358,6 +1432,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_interleave")) {
         *value = c->batch_interleave;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "dwt_pair_order")) {
+        *value = c->dwt_pair_order;
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_variant")) {

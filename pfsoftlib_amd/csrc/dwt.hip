@@ -150,18 +150,9 @@ __device__ __forceinline__ void init_chains(Chains<NCH>& ch, const ChainTables<N
     }
 }
 
-// Pair handled by workgroup b.  Workgroups are dealt to the 8 XCDs round-robin;
-// within each window of 64 pairs (heaviest first overall) this gives every XCD
-// a run of 8 consecutive pairs -- same M, consecutive M' -- whose coefficients
-// (l, M', M) etc. sit next to each other, so one L2 serves the neighbouring
-// reads and merges the neighbouring partial-line writes.  The last, partial
-// window keeps the identity order.
-__device__ __forceinline__ int xcd_pair(int b, int npairs)
-{
-    const int g = b >> 6, r = b & 63;
-    if ((g + 1) * 64 > npairs) return b;
-    return g * 64 + (r & 7) * 8 + (r >> 3);
-}
+// Workgroup b handles the pair t.xpairs[b]: the host orders the pairs for the
+// 8 XCDs (round-robin workgroup dispatch) so that neighbouring pairs, whose
+// coefficients share cache lines, run on one XCD at about the same time.
 
 // Orders one wave's LDS accesses around a point: its earlier LDS operations
 // complete before any later one (no other wave is waited for).
@@ -267,7 +258,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     constexpr int TS = tile_stride(NH);
     constexpr int WREG = fwd_wave_region(NH);
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
+    const int2 pr = t.xpairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;
@@ -434,7 +425,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     constexpr int TS = tile_stride(NCH);
     constexpr int WREG = inv_wave_region(NCH);  // a wave's tile, reused as its output staging
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
+    const int2 pr = t.xpairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;

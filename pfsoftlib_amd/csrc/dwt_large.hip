@@ -24,14 +24,7 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Pair handled by workgroup b: runs of 8 consecutive pairs per XCD within
-// windows of 64 (round-robin XCD dispatch), identity in the last partial window.
-__device__ __forceinline__ int xcd_pair(int b, int npairs)
-{
-    const int g = b >> 6, r = b & 63;
-    if ((g + 1) * 64 > npairs) return b;
-    return g * 64 + (r & 7) * 8 + (r >> 3);
-}
+// Workgroup b handles the pair t.xpairs[b] (an XCD-aware order built on the host).
 
 struct CaseInfo {
     long vec;     // pencil offset (complex elements) of the (p, q) beta-vector
@@ -198,7 +191,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_fwd_f64(PlanTables t, const do
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NH);
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
+    const int2 pr = t.xpairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;
@@ -332,7 +325,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64(PlanTables t, const do
     extern __shared__ __align__(16) double sm[];
     constexpr int TS = tile_stride(NCH);
     const int B = t.B, N = t.N;
-    const int2 pr = t.pairs[xcd_pair(blockIdx.x, t.npairs)];
+    const int2 pr = t.xpairs[blockIdx.x];
     const int M = pr.x, Mp = pr.y;
     const int R = B - M;
     const int nch = (R + 15) >> 4;

@@ -117,6 +117,7 @@ struct PlanTables {
     const double* wts;     // quadrature weights w_B(k)
     const double* normfac; // sqrt((2M+1)/2) * sqrt(C(2M, M-M')), [M*B + M'], M >= M' >= 0
     const int2* pairs;     // canonical (M, M'), M >= M' >= 0, heaviest first
+    const int2* xpairs;    // the same pairs in the FP64 DWT launch order (workgroup b runs xpairs[b])
     const double2* tw;     // exp(-2 pi i j / N), j < N
     const float2* tw32;    // same, rounded to FP32
 };

@@ -177,6 +177,29 @@ def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
         ctx.tune("dwt_variant", 0)
 
 
+def test_dwt_pair_orders_bit_identical(ctx, b128):
+    # the FP64 DWT kernels' pair launch order (per-XCD M' ranges, or runs of 8
+    # consecutive M' per window of 64) only moves pairs between workgroups: the
+    # results are the same bit for bit, at B = 128 and at pair counts that leave
+    # the last window partial (B = 5: 15 pairs, B = 20: 210 pairs) or the XCD
+    # queues uneven (B = 100: 5050 pairs)
+    B, c, g_ref, c_ref = b128
+    cases = [(B, c, g_ref)] + [(Bs, O.seeded_coeffs(Bs, 90 + Bs), None) for Bs in (5, 20, 100)]
+    try:
+        for Bs, cs, gs in cases:
+            if gs is None:
+                gs = O.idsoft(Bs, cs)
+            out = {}
+            for order in (0, 3):  # both directions in one order, then in the other
+                ctx.tune("dwt_pair_order", order)
+                out[order] = (ctx.idsoft(Bs, cs), ctx.dsoft(Bs, gs))
+            assert np.array_equal(out[0][0], out[3][0]) and np.array_equal(out[0][1], out[3][1]), Bs
+            assert rel(out[0][0], gs) < TOL
+            assert rel(out[0][1], cs) < 1e-11
+    finally:
+        ctx.tune("dwt_pair_order", 0)
+
+
 @pytest.mark.parametrize("B", [4, 32])
 def test_idsoft_reads_stale_nyquist_like_reference(ctx, B):
     N = 2 * B
