@@ -404,6 +404,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--fft-chunk", type=int, default=0, help="beta-slices per FFT chunk (0 = automatic)")
     ap.add_argument("--dwt-variant", type=int, default=0, help="work shape of the FP64 DWT kernels (0 = default)")
+    ap.add_argument("--dwt-inv-kernel", type=int, default=-1,
+                    help="inverse FP64 DWT kernel: 0 MFMA, 1..3 vector-ALU shapes (-1 = library default)")
     ap.add_argument("--dwt-pair-order", type=int, default=-1,
                     help="pair launch order of the FP64 DWT kernels, bit 0 forward, bit 1 inverse (0 = per-XCD M' "
                          "ranges, 1 = runs of 8 M' per XCD; -1 = library default)")
@@ -444,6 +446,8 @@ def main():
     ctx.tune("dwt_variant", args.dwt_variant)
     if args.dwt_pair_order >= 0:
         ctx.tune("dwt_pair_order", args.dwt_pair_order)
+    if args.dwt_inv_kernel >= 0:
+        ctx.tune("dwt_inv_kernel", args.dwt_inv_kernel)
     if args.batch_fft_signals >= 0:
         ctx.tune("batch_fft_signals", args.batch_fft_signals)
     if args.batch_fft_slices >= 0:

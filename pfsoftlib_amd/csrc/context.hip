@@ -269,6 +269,12 @@ struct pfs_context {
     long batch_chunk_signals = 64;
     long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
     long dwt_variant = 0;       // work shape of the FP64 DWT kernels (2B <= 256)
+    // inverse DWT kernel: 0 MFMA; vector ALU with 2 chains x 2 waves, 1 x 4
+    // or 4 x 1 per pair (2B <= 256; 4 x 2 at 2B = 512): 1..3 with separate
+    // broadcast moves, 5..7 with the broadcasts folded into the FMAs.
+    // Measured: 7 takes B = 128 from 120 to 109-113 us, B = 256 from 1.49 to
+    // 1.30 ms.
+    long dwt_inv_kernel = 7;
     // pair launch order of the FP64 DWT kernels (Plan::xorder): bit 0 the
     // forward's, bit 1 the inverse's.  Measured at B = 128: the per-XCD M'
     // ranges halve the inverse's HBM fetch (183 -> 96 MB) and cut the forward's
@@ -706,6 +712,9 @@ PlanTables dwt_tables(const pfs_context* c, const Plan& p, bool fwd)
     return t;
 }
 
+// Inverse DWT launch variant: the work shape in the low bits, the kernel kind above.
+int inv_variant(const pfs_context* c) { return (int)c->dwt_variant | (int)(c->dwt_inv_kernel << 8); }
+
 // pi / (B * 2B) and (B * 2B) / pi in double (src/fn_dsoft.cpp:119, src/fn_idsoft.cpp:109)
 inline double fwd_norm(int B) { return 3.14159265358979323846 / (B * (2 * B)); }
 inline double inv_norm(int B) { return (B * (2 * B)) / 3.14159265358979323846; }
@@ -741,7 +750,7 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     const bool fused = p->t.N <= 256;
     if (!fused) HIP_TRY(timed(c, "nyquist_gather", st, [&] { return launch_nyquist_gather_f64(grid, G, B, st, stale); }));
     HIP_TRY(timed(c, "dwt_inv", st, [&] {
-        return launch_dwt_inv_f64(dwt_tables(c, *p, false), coeffs, G, inv_norm(B), st, (int)c->dwt_variant, fused && !stale ? grid : nullptr,
+        return launch_dwt_inv_f64(dwt_tables(c, *p, false), coeffs, G, inv_norm(B), st, inv_variant(c), fused && !stale ? grid : nullptr,
                                   fused ? stale : nullptr);
     }));
     return fft_inv(c, *p, G, W, grid, st);
@@ -1326,7 +1335,7 @@ int pfs_stage_dwt_inv_dev(pfs_context* c, int B, const void* coeffs_dev, void* p
     int rc = get_plan(c, B, &p);
     if (rc) return rc;
     HIP_TRY(launch_dwt_inv_f64(dwt_tables(c, *p, false), (const double2*)coeffs_dev, (double2*)pencils_dev, inv_norm(B), pick(c, stream),
-                               (int)c->dwt_variant));
+                               inv_variant(c)));
     return PFS_OK;
 }
 
@@ -1376,6 +1385,12 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
     if (!strcmp(key, "batch_interleave")) {
         if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "batch_interleave must be 0 or 1");
         c->batch_interleave = value;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "dwt_inv_kernel")) {
+        if (value < 0 || value > 7 || value == 4)
+            return fail(PFS_ERR_ARG, "dwt_inv_kernel must be 0 (MFMA), 1..3 or 5..7 (vector ALU)");
+        c->dwt_inv_kernel = value;
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_pair_order")) {
@@ -1432,6 +1447,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_interleave")) {
         *value = c->batch_interleave;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "dwt_inv_kernel")) {
+        *value = c->dwt_inv_kernel;
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_pair_order")) {

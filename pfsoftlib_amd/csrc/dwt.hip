@@ -1,7 +1,8 @@
 // Discrete Wigner transform stage, FP64, Wigner-d generated on the fly.
 //
 // One wave per canonical pair (M, M'), M >= M' >= 0, for 2B <= 256 nodes
-// (dwt_large.hip takes larger grids).  Lane i of the wave runs the three-term recurrence of the
+// (dwt_large.hip takes the larger forward grids; the vector-ALU inverse
+// below, k_dwt_inv_f64_valu, the default inverse, also runs 2B = 512).  Lane i of the wave runs the three-term recurrence of the
 // L2-normalised Wigner d-function (fn_dwt.hpp:152-199: base row, then rows
 // l = M+1 .. B-1) in FP64 registers for the beta nodes k = i, i+64, ... --
 // one recurrence chain per 64 nodes.  Sixteen degrees of all chains at a time
@@ -547,6 +548,169 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
+// acc += (lane q of this lane's 16-lane row of c) * d, as one v_fmac_f64_dpp
+// (row_newbcast: DPP on 64-bit ALU ops; the compiler does not fold a separate
+// broadcast move into the FMA by itself).  The statements are volatile, so
+// they keep their program order: the first one after c is written (FIRST)
+// starts with the two wait states a DPP read of a VALU-written VGPR needs,
+// the same s_nop 1 the compiler places before its own DPP moves.
+#define PFS_FMAC_BCAST(Q)                                                                                   \
+    template <bool FIRST> __device__ __forceinline__ void fmac_bcast_##Q(double& acc, double c, double d)   \
+    {                                                                                                       \
+        if (FIRST)                                                                                          \
+            asm volatile("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:" #Q " row_mask:0xf bank_mask:0xf" \
+                         : "+v"(acc) : "v"(c), "v"(d));                                                     \
+        else                                                                                                \
+            asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #Q " row_mask:0xf bank_mask:0xf"           \
+                         : "+v"(acc) : "v"(c), "v"(d));                                                     \
+    }
+PFS_FMAC_BCAST(0) PFS_FMAC_BCAST(1) PFS_FMAC_BCAST(2) PFS_FMAC_BCAST(3)
+PFS_FMAC_BCAST(4) PFS_FMAC_BCAST(5) PFS_FMAC_BCAST(6) PFS_FMAC_BCAST(7)
+PFS_FMAC_BCAST(8) PFS_FMAC_BCAST(9) PFS_FMAC_BCAST(10) PFS_FMAC_BCAST(11)
+PFS_FMAC_BCAST(12) PFS_FMAC_BCAST(13) PFS_FMAC_BCAST(14) PFS_FMAC_BCAST(15)
+#undef PFS_FMAC_BCAST
+
+// q is a constant once the caller's loop is unrolled; the switch folds away.
+template <bool FIRST> __device__ __forceinline__ void fmac_bcast(int q, double& acc, double c, double d)
+{
+    switch (q) {
+    case 0: fmac_bcast_0<FIRST>(acc, c, d); break;
+    case 1: fmac_bcast_1<FIRST>(acc, c, d); break;
+    case 2: fmac_bcast_2<FIRST>(acc, c, d); break;
+    case 3: fmac_bcast_3<FIRST>(acc, c, d); break;
+    case 4: fmac_bcast_4<FIRST>(acc, c, d); break;
+    case 5: fmac_bcast_5<FIRST>(acc, c, d); break;
+    case 6: fmac_bcast_6<FIRST>(acc, c, d); break;
+    case 7: fmac_bcast_7<FIRST>(acc, c, d); break;
+    case 8: fmac_bcast_8<FIRST>(acc, c, d); break;
+    case 9: fmac_bcast_9<FIRST>(acc, c, d); break;
+    case 10: fmac_bcast_10<FIRST>(acc, c, d); break;
+    case 11: fmac_bcast_11<FIRST>(acc, c, d); break;
+    case 12: fmac_bcast_12<FIRST>(acc, c, d); break;
+    case 13: fmac_bcast_13<FIRST>(acc, c, d); break;
+    case 14: fmac_bcast_14<FIRST>(acc, c, d); break;
+    default: fmac_bcast_15<FIRST>(acc, c, d); break;
+    }
+}
+
+// Inverse DWT on the vector ALU.  In the inverse, lane k owns node k's
+// outputs outright: for each degree l, out_j(k) += d_l(k) * c_l,j over the 16
+// right-hand sides j (8 cases x re/im), and c_l,j is the same for every lane.
+// So the recurrence value never leaves its register: each FMA takes c_l,j
+// broadcast from lane j of its 16-lane row (DPP row_newbcast), no LDS tile,
+// no operand shuffles, no per-chunk syncs.  The FP64 vector and matrix peaks
+// of the device are equal, so the MFMA form has no arithmetic advantage here.
+//
+// Coefficients: every 16-lane row of ring[rr] holds the 16 values of degree
+// r = 16 chk + rr (lane j: case j/2, re/im j&1), so the four rows load the
+// same 128 bytes.  Slot rr is refilled with the next chunk's degree right
+// after its last use: each load has a whole chunk of FMAs to arrive.
+// Signs and the norm are per lane (case) and per degree parity: the d(pi -
+// beta) cases 4-7 carry (-1)^(l-M).  Workgroups past the last pair copy the
+// stale Nyquist entries, as in k_dwt_inv_f64.
+template <int NCH, int WPP, bool DPPFMA>
+__global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, const double2* __restrict__ coeffs,
+                                                               double2* __restrict__ G, double norm,
+                                                               const double2* __restrict__ nq_grid,
+                                                               const double2* __restrict__ nq_compact)
+{
+    if ((int)blockIdx.x >= t.npairs) {
+        nyquist_gather_one(nq_grid, G, t.B, nq_compact, (long)(blockIdx.x - t.npairs) * (64 * WPP) + threadIdx.x);
+        return;
+    }
+    extern __shared__ __align__(16) double sm[];
+    const int B = t.B, N = t.N;
+    const int2 pr = t.xpairs[blockIdx.x];
+    const int M = pr.x, Mp = pr.y;
+    const int R = B - M;
+    const int nch = (R + 15) >> 4;
+    const int RP = nch * 16;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int kw = w * 64 * NCH;
+    double* cf = sm;
+
+    ChainTables<NCH> ct;
+    load_chain_tables<NCH>(ct, t, kw + lane, false);
+    const double nf = t.normfac[M * B + Mp];
+
+    // this lane's right-hand side j = lane % 16: case j/2, part j&1
+    const int j = lane & 15;
+    const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
+    const double* csrc = reinterpret_cast<const double*>(coeffs) + (j & 1);
+    const double fac_even = lc.active ? norm * lc.sgn : 0.0;
+    const double fac_odd = lc.active ? norm * lc.sgn_odd : 0.0;
+    auto cidx = [&](int r) { return r < R ? 2 * coeff_index32(M + r, lc.Mc, lc.Mpc) : 0; };
+    double ring[16];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) ring[rr] = csrc[cidx(rr)];
+
+    for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+
+    double acc[NCH][16];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[c][q] = 0.0;
+    __syncthreads();  // the step-coefficient table is complete
+
+    for (int chk = 0; chk < nch; ++chk) {
+        const int rl = chk * 16 + (lane & 15);
+        const double vf1 = cf[rl], vf1f2 = cf[RP + rl], vc1 = cf[2 * RP + rl];
+        const int rn = (chk + 1) * 16;  // first degree of the next chunk
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            if (chk * 16 + rr < R) {  // uniform
+                const double cv = ring[rr] * ((rr & 1) ? fac_odd : fac_even);
+                const double f1 = row_bcast(vf1, rr), f1f2 = row_bcast(vf1f2, rr), c1 = row_bcast(vc1, rr);
+                if (DPPFMA) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q)
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c) {
+                            if (q == 0 && c == 0) fmac_bcast<true>(q, acc[c][q], cv, ch.cur[c]);
+                            else fmac_bcast<false>(q, acc[c][q], cv, ch.cur[c]);
+                        }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const double cq = row_bcast(cv, q);
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c) acc[c][q] = fma(ch.cur[c], cq, acc[c][q]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    const double u = fma(f1, ch.cb[c], f1f2);
+                    const double dn = fma(ch.cur[c], u, c1 * ch.prev[c]);
+                    ch.prev[c] = ch.cur[c];
+                    ch.cur[c] = dn;
+                }
+            }
+            ring[rr] = csrc[cidx(rn + rr)];
+        }
+    }
+
+    // each lane writes its nodes' values straight into the 8 pencils
+    // (1 KB contiguous per case and chain, reversed for cases 4-7)
+#pragma unroll
+    for (int cs = 0; cs < 8; ++cs) {
+        const CaseInfo co = case_info(cs, M, Mp, N);
+        if (!co.active) continue;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int k = kw + 64 * c + lane;
+            if (k < N) {
+                double2 val;
+                val.x = acc[c][2 * cs];
+                val.y = acc[c][2 * cs + 1];
+                G[co.vec + (co.flip ? N - 1 - k : k)] = val;
+            }
+        }
+    }
+}
+
 struct DwtShape {
     int nch, nh, wpp;
 };
@@ -613,7 +777,39 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     if (t.N > 1024) return hipErrorInvalidValue;
     if (t.N > 256) {
         if (nq_grid || nq_compact) return hipErrorInvalidValue;  // gathered by its own kernel there
-        return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
+        if (!((variant >> 8) & 3) || t.N > 512) return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
+    }
+    // bits 8-9 of variant: 0 = the MFMA kernel (shape from the low bits),
+    // 1..3 = the vector-ALU kernel with 2 chains x 2 waves, 1 x 4 or 4 x 1 per
+    // pair (grids of 2B <= 128 use the shape that covers their nodes); bit 10:
+    // its broadcasts folded into the FMAs
+    const int kind = (variant >> 8) & 3;
+    const bool dppfma = (variant >> 10) & 1;  // the broadcast folded into each FMA
+    variant &= 0xff;
+    if (kind) {
+        // 2B = 512: four chains x two waves per pair, whatever the kind
+        const int nch = t.N <= 128 ? 1 : t.N > 256 ? 4 : (kind == 1 ? 2 : kind == 2 ? 1 : 4);
+        const int wpp = t.N <= 64 ? 1 : t.N <= 128 ? 2 : t.N > 256 ? 2 : (kind == 1 ? 2 : kind == 2 ? 4 : 1);
+        const size_t smem = 3 * (size_t)(((t.B + 15) / 16) * 16) * sizeof(double);
+        const long nq_total = (nq_grid || nq_compact) ? (long)(2 * t.N - 1) * t.N : 0;
+        const int nq_blocks = (int)((nq_total + 64 * wpp - 1) / (64 * wpp));
+        dim3 grid(t.npairs + nq_blocks), block(64 * wpp);
+#define PFS_INVV(NC, W)                                                                                     \
+        if (nch == NC && wpp == W) {                                                                        \
+            if (dppfma)                                                                                     \
+                return pfs::launch((k_dwt_inv_f64_valu<NC, W, true>), grid, block, smem, st, t, coeffs, G, norm, \
+                                   nq_grid, nq_compact);                                                    \
+            return pfs::launch((k_dwt_inv_f64_valu<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm,   \
+                               nq_grid, nq_compact);                                                        \
+        }
+        PFS_INVV(1, 1)
+        PFS_INVV(1, 2)
+        PFS_INVV(2, 2)
+        PFS_INVV(1, 4)
+        PFS_INVV(4, 1)
+        PFS_INVV(4, 2)
+#undef PFS_INVV
+        return hipErrorInvalidValue;
     }
     const DwtShape s = dwt_shape(t.N, false, variant);
     const size_t smem = dwt_smem(t.B, t.N, false, variant);

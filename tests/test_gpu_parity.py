@@ -14,6 +14,7 @@ import oracle_bind as O
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-11
+DEFAULT_INV_KERNEL = 7  # the library's default inverse DWT kernel (tuning key dwt_inv_kernel)
 
 
 def rel(a, b):
@@ -177,6 +178,33 @@ def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
         ctx.tune("dwt_variant", 0)
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 5, 6, 7])
+def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
+    # the MFMA inverse DWT (0) and the vector-ALU one in its three work shapes
+    # (1: 2 chains x 2 waves, 2: 1 x 4, 3: 4 x 1 per pair; 5-7 the same with the
+    # row broadcasts folded into the FMAs; grids of 2B <= 128
+    # use the shape that covers their nodes), at B = 128, at grids that leave
+    # part of a shape's nodes empty (2B = 40, 80, 200), at B = 1..4, and through
+    # the device route with stale Nyquist entries (copied by the same launch)
+    B, c, g_ref, _ = b128
+    ctx.tune("dwt_inv_kernel", kind)
+    try:
+        assert rel(ctx.idsoft(B, c), g_ref) < TOL
+        for Bs in (1, 2, 3, 4, 20, 40, 100):
+            cs = O.seeded_coeffs(Bs, 300 + Bs)
+            assert rel(ctx.idsoft(Bs, cs), O.idsoft(Bs, cs)) < TOL, Bs
+        Bs = 64
+        N = 2 * Bs
+        cs = O.seeded_coeffs(Bs, 6)
+        stale = np.random.default_rng(7).standard_normal(N ** 3) + 0j
+        gd = torch.from_numpy(stale.copy()).cuda()
+        ctx.idsoft_dev(Bs, torch.from_numpy(cs).cuda(), gd, stream=torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        assert rel(gd.cpu().numpy(), O.idsoft(Bs, cs, grid=stale)) < TOL
+    finally:
+        ctx.tune("dwt_inv_kernel", DEFAULT_INV_KERNEL)
+
+
 def test_dwt_pair_orders_bit_identical(ctx, b128):
     # the FP64 DWT kernels' pair launch order (per-XCD M' ranges, or runs of 8
     # consecutive M' per window of 64) only moves pairs between workgroups: the
@@ -300,6 +328,32 @@ def test_b256_roundtrip_and_linearity(ctx, torch):
     lin = g1 + 2 * g2
     torch.cuda.synchronize()
     assert (torch.linalg.vector_norm(g - lin) / torch.linalg.vector_norm(lin)).item() < 1e-13
+
+
+def test_b256_inverse_kernels_agree(ctx, torch):
+    # 2B = 512: the vector-ALU inverse DWT (four chains x two waves per pair)
+    # against the large-grid MFMA kernel on the same coefficients (different
+    # summation order, so to rounding), and its grid back through DSOFT
+    B, N = 256, 512
+    import pfsoftlib_amd as P
+    s = torch.cuda.current_stream()
+    c = torch.from_numpy(P.seeded_coeffs(B, 9, 1.5)).cuda()
+    out = {}
+    try:
+        for kind in (0, 7):
+            ctx.tune("dwt_inv_kernel", kind)
+            g = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")
+            ctx.idsoft_dev(B, c, g, stream=s)
+            out[kind] = g
+    finally:
+        ctx.tune("dwt_inv_kernel", DEFAULT_INV_KERNEL)
+    nrm = torch.linalg.vector_norm
+    assert (nrm(out[7] - out[0]) / nrm(out[0])).item() < 1e-13
+    r = torch.empty(O.coeff_count(B), dtype=torch.complex128, device="cuda")
+    ctx.dsoft_dev(B, out[7], r, stream=s)
+    torch.cuda.synchronize()
+    assert (nrm(r - c) / nrm(c)).item() < TOL
+    del out, r
 
 
 def test_cpp_dropin_example_like_reference(ctx):
