@@ -161,7 +161,9 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *   "dwt_inv_kernel"    inverse FP64 DWT kernel: 7 (default) the vector-ALU
  *                       kernel with its row broadcasts folded into the FMAs,
  *                       0 the MFMA kernel; 1..3 / 5..7 select vector-ALU work
- *                       shapes (results agree to rounding).
+ *                       shapes (2 chains x 2 waves over the nodes, 4 x 2 over
+ *                       the right-hand sides, 4 x 1; 5..7 with the folded
+ *                       broadcasts; results agree to rounding).
  *   "dwt_pair_order"    pair launch order of the FP64 DWT kernels, bit 0 the
  *                       forward's, bit 1 the inverse's: 0 = every XCD takes a
  *                       contiguous M' range of each M (default, both), 1 = runs

@@ -269,9 +269,10 @@ struct pfs_context {
     long batch_chunk_signals = 64;
     long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
     long dwt_variant = 0;       // work shape of the FP64 DWT kernels (2B <= 256)
-    // inverse DWT kernel: 0 MFMA; vector ALU with 2 chains x 2 waves, 1 x 4
-    // or 4 x 1 per pair (2B <= 256; 4 x 2 at 2B = 512): 1..3 with separate
-    // broadcast moves, 5..7 with the broadcasts folded into the FMAs.
+    // inverse DWT kernel: 0 MFMA; vector ALU with 2 chains x 2 waves over the
+    // nodes, 4 chains x 2 waves over the right-hand sides, or 4 x 1 per pair
+    // (2B <= 256; 4 x 2 at 2B = 512): 1..3 with separate broadcast moves,
+    // 5..7 with the broadcasts folded into the FMAs.
     // Measured: 7 takes B = 128 from 120 to 109-113 us, B = 256 from 1.49 to
     // 1.30 ms.
     long dwt_inv_kernel = 7;

@@ -33,6 +33,8 @@
 // multiplies the base row and propagates through the linear recurrence.
 #include "pfs_internal.h"
 
+#include <type_traits>
+
 namespace pfs {
 
 constexpr int OSTR = 18;            // output staging row stride (doubles), 64 rows
@@ -608,7 +610,11 @@ template <bool FIRST> __device__ __forceinline__ void fmac_bcast(int q, double& 
 // Signs and the norm are per lane (case) and per degree parity: the d(pi -
 // beta) cases 4-7 carry (-1)^(l-M).  Workgroups past the last pair copy the
 // stale Nyquist entries, as in k_dwt_inv_f64.
-template <int NCH, int WPP, bool DPPFMA>
+// QS = 1: the WPP waves of a pair split its nodes.  QS = 2 (WPP = 2): both
+// waves run all NCH x 64 nodes and each takes 8 of the 16 right-hand sides,
+// which halves the accumulators (one more wave per SIMD) at the price of
+// running the recurrence twice.
+template <int NCH, int WPP, bool DPPFMA, int QS>
 __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, const double2* __restrict__ coeffs,
                                                                double2* __restrict__ G, double norm,
                                                                const double2* __restrict__ nq_grid,
@@ -626,7 +632,9 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
     const int nch = (R + 15) >> 4;
     const int RP = nch * 16;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int kw = w * 64 * NCH;
+    constexpr int QW = 16 / QS;               // right-hand sides per wave
+    const int kw = QS == 1 ? w * 64 * NCH : 0;
+    const int q0 = QS == 1 ? 0 : QW * w;      // this wave's first right-hand side
     double* cf = sm;
 
     ChainTables<NCH> ct;
@@ -648,12 +656,34 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
     Chains<NCH> ch;
     init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
 
-    double acc[NCH][16];
+    double acc[NCH][QW];
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) acc[c][q] = 0.0;
+        for (int q = 0; q < QW; ++q) acc[c][q] = 0.0;
     __syncthreads();  // the step-coefficient table is complete
+
+    // acc[c][q - Q0] += c_q * d(c) for this wave's right-hand sides q (Q0 is a
+    // constant, so every broadcast lane is an immediate)
+    auto contract = [&](auto q0c, double cv) {
+        constexpr int Q0 = decltype(q0c)::value;
+        if (DPPFMA) {
+#pragma unroll
+            for (int q = Q0; q < Q0 + QW; ++q)
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    if (q == Q0 && c == 0) fmac_bcast<true>(q, acc[c][q - Q0], cv, ch.cur[c]);
+                    else fmac_bcast<false>(q, acc[c][q - Q0], cv, ch.cur[c]);
+                }
+        } else {
+#pragma unroll
+            for (int q = Q0; q < Q0 + QW; ++q) {
+                const double cq = row_bcast(cv, q);
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) acc[c][q - Q0] = fma(ch.cur[c], cq, acc[c][q - Q0]);
+            }
+        }
+    };
 
     for (int chk = 0; chk < nch; ++chk) {
         const int rl = chk * 16 + (lane & 15);
@@ -664,22 +694,8 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
             if (chk * 16 + rr < R) {  // uniform
                 const double cv = ring[rr] * ((rr & 1) ? fac_odd : fac_even);
                 const double f1 = row_bcast(vf1, rr), f1f2 = row_bcast(vf1f2, rr), c1 = row_bcast(vc1, rr);
-                if (DPPFMA) {
-#pragma unroll
-                    for (int q = 0; q < 16; ++q)
-#pragma unroll
-                        for (int c = 0; c < NCH; ++c) {
-                            if (q == 0 && c == 0) fmac_bcast<true>(q, acc[c][q], cv, ch.cur[c]);
-                            else fmac_bcast<false>(q, acc[c][q], cv, ch.cur[c]);
-                        }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const double cq = row_bcast(cv, q);
-#pragma unroll
-                        for (int c = 0; c < NCH; ++c) acc[c][q] = fma(ch.cur[c], cq, acc[c][q]);
-                    }
-                }
+                if (q0 == 0) contract(std::integral_constant<int, 0>(), cv);
+                else contract(std::integral_constant<int, QW>(), cv);  // QS = 2, second wave
 #pragma unroll
                 for (int c = 0; c < NCH; ++c) {
                     const double u = fma(f1, ch.cb[c], f1f2);
@@ -695,7 +711,8 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
     // each lane writes its nodes' values straight into the 8 pencils
     // (1 KB contiguous per case and chain, reversed for cases 4-7)
 #pragma unroll
-    for (int cs = 0; cs < 8; ++cs) {
+    for (int cl = 0; cl < QW / 2; ++cl) {
+        const int cs = q0 / 2 + cl;
         const CaseInfo co = case_info(cs, M, Mp, N);
         if (!co.active) continue;
 #pragma unroll
@@ -703,8 +720,8 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
             const int k = kw + 64 * c + lane;
             if (k < N) {
                 double2 val;
-                val.x = acc[c][2 * cs];
-                val.y = acc[c][2 * cs + 1];
+                val.x = acc[c][2 * cl];
+                val.y = acc[c][2 * cl + 1];
                 G[co.vec + (co.flip ? N - 1 - k : k)] = val;
             }
         }
@@ -780,7 +797,8 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
         if (!((variant >> 8) & 3) || t.N > 512) return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
     }
     // bits 8-9 of variant: 0 = the MFMA kernel (shape from the low bits),
-    // 1..3 = the vector-ALU kernel with 2 chains x 2 waves, 1 x 4 or 4 x 1 per
+    // 1..3 = the vector-ALU kernel with 2 chains x 2 waves over the nodes,
+    // 4 chains x 2 waves over the right-hand sides, or 4 chains x 1 wave per
     // pair (grids of 2B <= 128 use the shape that covers their nodes); bit 10:
     // its broadcasts folded into the FMAs
     const int kind = (variant >> 8) & 3;
@@ -788,26 +806,27 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     variant &= 0xff;
     if (kind) {
         // 2B = 512: four chains x two waves per pair, whatever the kind
-        const int nch = t.N <= 128 ? 1 : t.N > 256 ? 4 : (kind == 1 ? 2 : kind == 2 ? 1 : 4);
-        const int wpp = t.N <= 64 ? 1 : t.N <= 128 ? 2 : t.N > 256 ? 2 : (kind == 1 ? 2 : kind == 2 ? 4 : 1);
+        const int nch = t.N <= 128 ? 1 : t.N > 256 ? 4 : (kind == 1 ? 2 : 4);
+        const int wpp = t.N <= 64 ? 1 : t.N <= 128 ? 2 : t.N > 256 ? 2 : (kind == 3 ? 1 : 2);
+        const int qs = (t.N > 128 && t.N <= 256 && kind == 2) ? 2 : 1;
         const size_t smem = 3 * (size_t)(((t.B + 15) / 16) * 16) * sizeof(double);
         const long nq_total = (nq_grid || nq_compact) ? (long)(2 * t.N - 1) * t.N : 0;
         const int nq_blocks = (int)((nq_total + 64 * wpp - 1) / (64 * wpp));
         dim3 grid(t.npairs + nq_blocks), block(64 * wpp);
-#define PFS_INVV(NC, W)                                                                                     \
-        if (nch == NC && wpp == W) {                                                                        \
+#define PFS_INVV(NC, W, QS)                                                                                 \
+        if (nch == NC && wpp == W && qs == QS) {                                                            \
             if (dppfma)                                                                                     \
-                return pfs::launch((k_dwt_inv_f64_valu<NC, W, true>), grid, block, smem, st, t, coeffs, G, norm, \
-                                   nq_grid, nq_compact);                                                    \
-            return pfs::launch((k_dwt_inv_f64_valu<NC, W, false>), grid, block, smem, st, t, coeffs, G, norm,   \
-                               nq_grid, nq_compact);                                                        \
+                return pfs::launch((k_dwt_inv_f64_valu<NC, W, true, QS>), grid, block, smem, st, t, coeffs, G, \
+                                   norm, nq_grid, nq_compact);                                              \
+            return pfs::launch((k_dwt_inv_f64_valu<NC, W, false, QS>), grid, block, smem, st, t, coeffs, G,    \
+                               norm, nq_grid, nq_compact);                                                  \
         }
-        PFS_INVV(1, 1)
-        PFS_INVV(1, 2)
-        PFS_INVV(2, 2)
-        PFS_INVV(1, 4)
-        PFS_INVV(4, 1)
-        PFS_INVV(4, 2)
+        PFS_INVV(1, 1, 1)
+        PFS_INVV(1, 2, 1)
+        PFS_INVV(2, 2, 1)
+        PFS_INVV(4, 1, 1)
+        PFS_INVV(4, 2, 1)
+        PFS_INVV(4, 2, 2)
 #undef PFS_INVV
         return hipErrorInvalidValue;
     }

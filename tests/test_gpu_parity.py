@@ -181,7 +181,8 @@ def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
 @pytest.mark.parametrize("kind", [0, 1, 2, 3, 5, 6, 7])
 def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
     # the MFMA inverse DWT (0) and the vector-ALU one in its three work shapes
-    # (1: 2 chains x 2 waves, 2: 1 x 4, 3: 4 x 1 per pair; 5-7 the same with the
+    # (1: 2 chains x 2 waves over the nodes, 2: 4 chains x 2 waves over the
+    # right-hand sides, 3: 4 x 1 per pair; 5-7 the same with the
     # row broadcasts folded into the FMAs; grids of 2B <= 128
     # use the shape that covers their nodes), at B = 128, at grids that leave
     # part of a shape's nodes empty (2B = 40, 80, 200), at B = 1..4, and through
