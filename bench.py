@@ -117,6 +117,36 @@ def cpu_baseline(B, pairs, seed):
     }
 
 
+def cpu_other_configs(seed):
+    """The oracle on this host at the other CPU-runnable workloads, all host
+    threads, bounded samples: the B=32 round trip (five round trips), and the
+    512-signal B=64 batch as sequential complex128 round trips (four signals
+    of the 512; the CPU path has no complex64 or batched form)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import numpy as np
+    import oracle_bind as O
+    threads = O.lib().oracle_max_threads()
+    res = {}
+    for name, B, n in (("b32_roundtrip", 32, 5), ("b64_batch_sample", 64, 4)):
+        t_inv = t_fwd = 0.0
+        err = 0.0
+        for i in range(n):
+            c = O.seeded_coeffs(B, seed + i)
+            t0 = time.perf_counter()
+            g = O.idsoft(B, c, threads=threads)
+            t1 = time.perf_counter()
+            r = O.dsoft(B, g, threads=threads)
+            t2 = time.perf_counter()
+            t_inv += t1 - t0
+            t_fwd += t2 - t1
+            err = max(err, float(np.linalg.norm(r - c) / np.linalg.norm(c)))
+        res[name] = {"bandwidth": B, "pairs": n, "pairs_per_s": n / (t_inv + t_fwd), "cores": threads,
+                     "forward_s_avg": t_fwd / n, "inverse_s_avg": t_inv / n, "roundtrip_rel_l2_max": err}
+    res["note"] = ("oracle (C/OpenMP restatement of the reference), complex128; b64_batch_sample: %d of the 512 "
+                   "signals of the batched B=64 workload, one after another" % res["b64_batch_sample"]["pairs"])
+    return res
+
+
 def batch_dwt_flops(B):
     """Useful FP32 flops of one signal's batched DWT in one direction: every
     active (pair, case) contracts R = B - M degrees against N = 2B betas for
@@ -588,6 +618,7 @@ def main():
     out["dwt_accuracy"] = dwt_accuracy(ctx, np, 256, 10, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
+        out["cpu_other_configs"] = cpu_other_configs(args.seed)
     print(json.dumps(out))
     ctx.close()
 
