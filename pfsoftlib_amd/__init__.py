@@ -180,6 +180,11 @@ class Context:
     def tune(self, key, value):
         _check(lib().pfs_tune_set(self._h, key.encode(), int(value)))
 
+    def tune_get(self, key):
+        v = ctypes.c_long()
+        _check(lib().pfs_tune_get(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
+
     def timing(self, on, only=None):
         """Per-kernel timing on or off; only: kernel names to time (None: all)."""
         _check(lib().pfs_timing_select(self._h, ",".join(only).encode() if only else None))

@@ -14,7 +14,6 @@ import oracle_bind as O
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-11
-DEFAULT_INV_KERNEL = 7  # the library's default inverse DWT kernel (tuning key dwt_inv_kernel)
 
 
 def rel(a, b):
@@ -188,6 +187,7 @@ def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
     # part of a shape's nodes empty (2B = 40, 80, 200), at B = 1..4, and through
     # the device route with stale Nyquist entries (copied by the same launch)
     B, c, g_ref, _ = b128
+    default = ctx.tune_get("dwt_inv_kernel")
     ctx.tune("dwt_inv_kernel", kind)
     try:
         assert rel(ctx.idsoft(B, c), g_ref) < TOL
@@ -203,7 +203,7 @@ def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
         torch.cuda.synchronize()
         assert rel(gd.cpu().numpy(), O.idsoft(Bs, cs, grid=stale)) < TOL
     finally:
-        ctx.tune("dwt_inv_kernel", DEFAULT_INV_KERNEL)
+        ctx.tune("dwt_inv_kernel", default)
 
 
 def test_dwt_pair_orders_bit_identical(ctx, b128):
@@ -213,6 +213,7 @@ def test_dwt_pair_orders_bit_identical(ctx, b128):
     # the last window partial (B = 5: 15 pairs, B = 20: 210 pairs) or the XCD
     # queues uneven (B = 100: 5050 pairs)
     B, c, g_ref, c_ref = b128
+    default = ctx.tune_get("dwt_pair_order")
     cases = [(B, c, g_ref)] + [(Bs, O.seeded_coeffs(Bs, 90 + Bs), None) for Bs in (5, 20, 100)]
     try:
         for Bs, cs, gs in cases:
@@ -226,7 +227,7 @@ def test_dwt_pair_orders_bit_identical(ctx, b128):
             assert rel(out[0][0], gs) < TOL
             assert rel(out[0][1], cs) < 1e-11
     finally:
-        ctx.tune("dwt_pair_order", 0)
+        ctx.tune("dwt_pair_order", default)
 
 
 @pytest.mark.parametrize("B", [4, 32])
@@ -340,6 +341,7 @@ def test_b256_inverse_kernels_agree(ctx, torch):
     s = torch.cuda.current_stream()
     c = torch.from_numpy(P.seeded_coeffs(B, 9, 1.5)).cuda()
     out = {}
+    default = ctx.tune_get("dwt_inv_kernel")
     try:
         for kind in (0, 7):
             ctx.tune("dwt_inv_kernel", kind)
@@ -347,7 +349,7 @@ def test_b256_inverse_kernels_agree(ctx, torch):
             ctx.idsoft_dev(B, c, g, stream=s)
             out[kind] = g
     finally:
-        ctx.tune("dwt_inv_kernel", DEFAULT_INV_KERNEL)
+        ctx.tune("dwt_inv_kernel", default)
     nrm = torch.linalg.vector_norm
     assert (nrm(out[7] - out[0]) / nrm(out[0])).item() < 1e-13
     r = torch.empty(O.coeff_count(B), dtype=torch.complex128, device="cuda")
