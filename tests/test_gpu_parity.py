@@ -359,6 +359,32 @@ def test_b256_inverse_kernels_agree(ctx, torch):
     del out, r
 
 
+def test_b256_stale_nyquist_on_both_routes(ctx, torch):
+    # 2B = 512 (separate gather kernel, then the inverse DWT): with zero
+    # coefficients the inverse is the inverse FFT of the caller's entries at
+    # row B or column B alone, every other entry of the given grid ignored --
+    # slice k of the output is ifft2 of slice k masked to that row and column
+    B, N = 256, 512
+    gen = torch.Generator(device="cuda").manual_seed(256)
+    stale = torch.randn(N ** 3, dtype=torch.complex128, device="cuda", generator=gen)
+    zero = torch.zeros(O.coeff_count(B), dtype=torch.complex128, device="cuda")
+    s = torch.cuda.current_stream()
+    dev = stale.clone()
+    ctx.idsoft_dev(B, zero, dev, stream=s)
+    torch.cuda.synchronize()
+    host = ctx.idsoft(B, np.zeros(O.coeff_count(B), complex), grid=stale.cpu().numpy())
+    st3, d3, h3 = stale.view(N, N, N), dev.view(N, N, N), host.reshape(N, N, N)
+    for k in (0, 255, 511):
+        a = st3[k].cpu().numpy()                      # [c][r]
+        z = np.zeros_like(a)
+        z[B, :] = a[B, :]
+        z[:, B] = a[:, B]
+        ref = np.fft.ifft2(z)
+        assert rel(d3[k].cpu().numpy(), ref) < 1e-13
+        assert rel(h3[k], ref) < 1e-13
+    del stale, dev, host
+
+
 def test_cpp_dropin_example_like_reference(ctx):
     # tests/cpp/example_roundtrip.cpp mirrors examples/example.cpp through include/pfsoft_amd.hpp
     import os
