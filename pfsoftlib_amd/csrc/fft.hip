@@ -385,6 +385,8 @@ template <typename T, int KIND> constexpr int r16_lines()
 // measured 3-9 % faster so; the complex64 row stores (inverse passes) and all
 // complex128 stores stay ordinary (for one complex128 grid the next kernel
 // reads part of the output from the Infinity Cache).
+// (The 512-point kernel, k_fft_lines_r16x2, stores non-temporally in every
+// pass: at 2B = 512 one complex128 grid is 2.1 GB, eight times the cache.)
 template <typename T, int KIND> constexpr bool r16_nt_store()
 {
     return sizeof(T) == 4 && KIND == LINES_ROW_IN_TILE_OUT;
@@ -594,7 +596,7 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * 32)) void k_fft_lines_r16x2
         C r;
         r.x = (h ? theirs.x - mine.x : mine.x + theirs.x) * scale;
         r.y = (h ? theirs.y - mine.y : mine.y + theirs.y) * scale;
-        gstore<C, r16_nt_store<T, KIND>()>(dst, o + k * step, r);
+        gstore<C, true>(dst, o + k * step, r);  // 2B = 512: the grid is far beyond the Infinity Cache
     }
 }
 
