@@ -709,7 +709,11 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
     }
 
     // each lane writes its nodes' values straight into the 8 pencils
-    // (1 KB contiguous per case and chain, reversed for cases 4-7)
+    // (1 KB contiguous per case and chain, reversed for cases 4-7).  Up to
+    // 2B = 256 the stores stay ordinary: the next FFT pass finds part of the
+    // 268 MB pencil buffer in the Infinity Cache.  At 2B = 512 (2.1 GB) it
+    // cannot, and the stores are non-temporal.
+    const bool nts = N > 256;
 #pragma unroll
     for (int cl = 0; cl < QW / 2; ++cl) {
         const int cs = q0 / 2 + cl;
@@ -719,10 +723,12 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
         for (int c = 0; c < NCH; ++c) {
             const int k = kw + 64 * c + lane;
             if (k < N) {
-                double2 val;
+                d2v val;
                 val.x = acc[c][2 * cl];
                 val.y = acc[c][2 * cl + 1];
-                G[co.vec + (co.flip ? N - 1 - k : k)] = val;
+                d2v* dst = reinterpret_cast<d2v*>(G + co.vec + (co.flip ? N - 1 - k : k));
+                if (nts) __builtin_nontemporal_store(val, dst);
+                else *dst = val;
             }
         }
     }
