@@ -20,6 +20,8 @@
 
 namespace pfs {
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+
 int g_mfma_f32_layout = 0;
 
 namespace {
@@ -159,8 +161,14 @@ __device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC>& S, const float* 
 #pragma unroll
     for (int ch = 0; ch < NC; ++ch) S.a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
     const int e = e0 + kb * de;
+    // each pencil value is read once: a non-temporal load does not allocate, so
+    // the streamed pencils (8.6 GB for 64 grids at 2B = 256) do not evict the
+    // table tiles the other signal groups of the pair read from L2
 #pragma unroll
-    for (int sp = 0; sp < SPW; ++sp) S.g[sp] = *reinterpret_cast<const float4*>(gp[sp] + e);
+    for (int sp = 0; sp < SPW; ++sp) {
+        const v4f g = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(gp[sp] + e));
+        S.g[sp] = make_float4(g.x, g.y, g.z, g.w);
+    }
     S.w = *reinterpret_cast<const float4*>(wl + 16 * kb);
 }
 
@@ -438,8 +446,13 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
             const BCase co = bcase(c, M, Mp, NN);
             if (!co.active) continue;
             float2* dst = G + (long)s * N3 + co.vec;
-            for (int k = lane; k < NN; k += 64)
-                dst[co.flip ? NN - 1 - k : k] = make_float2(sw[k * OSTRF + 2 * c], sw[k * OSTRF + 2 * c + 1]);
+            // non-temporal: a batch's pencils are far beyond the Infinity Cache
+            for (int k = lane; k < NN; k += 64) {
+                v2f val;
+                val.x = sw[k * OSTRF + 2 * c];
+                val.y = sw[k * OSTRF + 2 * c + 1];
+                __builtin_nontemporal_store(val, reinterpret_cast<v2f*>(dst + (co.flip ? NN - 1 - k : k)));
+            }
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
