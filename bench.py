@@ -58,23 +58,43 @@ def measured_traffic(B, kernel):
 
 
 def cpu_model():
+    return cpu_info().get("model name", "unknown")
+
+
+def cpu_info():
+    """The first processor's /proc/cpuinfo fields that name the host CPU, plus
+    the count of logical processors and the threads this process may use."""
+    keep = ("vendor_id", "cpu family", "model", "model name", "stepping", "microcode", "cpu MHz", "cache size",
+            "siblings", "cpu cores", "address sizes")
+    out, n = {}, 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k == "processor":
+                    n += 1
+                elif n == 1 and k in keep:
+                    out[k] = v
     except OSError:
         pass
-    return "unknown"
+    out["logical_processors"] = n
+    try:
+        out["usable_threads"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return out
 
 
-def cpu_baseline(B, pairs, seed):
+def cpu_baseline(B, pairs, seed, sweep_B=64):
     """The oracle (C/OpenMP restatement of the reference's DSOFT/IDSOFT) on this
     host, following the reference's benchmark protocol
     (benchmark/benchmark_dsoft_for.cpp:44-48,135-223): warm the OpenMP pool,
     build the input once, time forward and inverse separately over `pairs`
     round trips with all host threads (avg/min/max), then one round trip with
-    one thread for the serial time, speedup and parallel efficiency."""
+    one thread for the serial time, speedup and parallel efficiency; then the
+    reference's thread sweep at bandwidth sweep_B (thread_sweep)."""
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
     import numpy as np
     import oracle_bind as O
@@ -99,6 +119,7 @@ def cpu_baseline(B, pairs, seed):
     s_inv, s_fwd, _ = run(1, 1)
     par = (sum(t_inv) + sum(t_fwd)) / pairs
     ser = s_inv[0] + s_fwd[0]
+    sweep = thread_sweep(O, sweep_B, threads, seed) if sweep_B else None
     return {
         "value": 1.0 / par,
         "unit": "DSOFT+iDSOFT pairs/s",
@@ -114,6 +135,41 @@ def cpu_baseline(B, pairs, seed):
         "speedup": ser / par,
         "efficiency": ser / par / threads,
         "roundtrip_rel_l2": err,
+        "cpu_info": cpu_info(),
+        "thread_sweep": sweep,
+    }
+
+
+def thread_sweep(O, B, max_threads, seed, runs=2):
+    """The reference's speedup and efficiency drivers
+    (benchmark/benchmark_dsoft_for_speedup.cpp:51-160,
+    benchmark_dsoft_for_efficiency.cpp:160): forward and inverse at every
+    thread count 1..max_threads (the serial run first), `runs` round trips
+    each, speedup = serial / t and efficiency = serial / (threads * t) per
+    direction.  Run at a bandwidth small enough that the whole sweep stays a
+    few seconds of host time."""
+    c = O.seeded_coeffs(B, seed)
+    g = O.idsoft(B, c)
+    rows = []
+    for th in range(1, max_threads + 1):
+        tf = ti = 0.0
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            O.idsoft(B, c, threads=th)
+            t1 = time.perf_counter()
+            O.dsoft(B, g, threads=th)
+            t2 = time.perf_counter()
+            ti += t1 - t0
+            tf += t2 - t1
+        rows.append((th, tf / runs, ti / runs))
+    sf, si = rows[0][1], rows[0][2]
+    return {
+        "bandwidth": B, "runs_per_count": runs,
+        "threads": [r[0] for r in rows],
+        "forward_s": [round(r[1], 6) for r in rows], "inverse_s": [round(r[2], 6) for r in rows],
+        "forward_speedup": [round(sf / r[1], 3) for r in rows], "inverse_speedup": [round(si / r[2], 3) for r in rows],
+        "forward_efficiency": [round(sf / r[1] / r[0], 3) for r in rows],
+        "inverse_efficiency": [round(si / r[2] / r[0], 3) for r in rows],
     }
 
 

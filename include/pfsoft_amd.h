@@ -38,7 +38,9 @@ enum pfs_status {
     PFS_ERR_DEVICE = 3,     /* HIP runtime error                                */
     PFS_ERR_MEMORY = 4,     /* device allocation failed                         */
     PFS_ERR_UNSUPPORTED = 5,/* bandwidth or size outside the supported range    */
-    PFS_ERR_BUDGET = 6      /* request does not fit the context's HBM budget    */
+    PFS_ERR_BUDGET = 6,     /* request does not fit the context's HBM budget    */
+    PFS_ERR_TIMEOUT = 7     /* a wait ended before the request finished; the
+                               request is still queued or running          */
 };
 
 enum pfs_precision {
@@ -126,6 +128,25 @@ int pfs_batch_shape(pfs_context* ctx, int bandwidth, int forward, int nsig, int*
  * process-wide context on device 0. */
 void pfs_layer_wise_DFT2_grid3D(int cols, int rows, int lays, double* arr, int threads);
 void pfs_layer_wise_IDFT2_grid3D(int cols, int rows, int lays, double* arr, int threads);
+
+/* The same two functions under the reference's own symbol names
+ * (include/PFSOFTlib_headers/fftw_wrapper.hpp:31-32, declared extern "C"
+ * inside namespace pfsoft, so the linker names carry no namespace): a
+ * reference build that links -lpfsoft_amd instead of src/fftw_wrapper.cpp +
+ * libfftw3 resolves grid3D::layer_wise_DFT2/IDFT2 (grid3D_cx.hpp:339,400)
+ * to these. */
+void uzl_fftw_layer_wise_DFT2_grid3D(int cols, int rows, int lays, double* arr, int threads);
+void uzl_fftw_layer_wise_IDFT2_grid3D(int cols, int rows, int lays, double* arr, int threads);
+
+/* grid3D<complex<double>>::layer_wise_DFT2 / layer_wise_IDFT2(scale, threads)
+ * (grid3D_cx.hpp:81-82, :309-366, :370-427) on host memory: the seam's
+ * transform (sign -1 forward, +1 backward), then every element multiplied by
+ * the complex scale (scale_re, scale_im), as the reference's serial loop
+ * does after the FFTW call; the scale is applied on the device.  Returns a
+ * status (the reference returns void).  Uses the process-wide context of the
+ * seam. */
+int pfs_layer_wise_dft2(int cols, int rows, int lays, double* arr, int sign, double scale_re, double scale_im,
+                        int threads);
 
 /* Device form of the seam: in place on arr_dev, result multiplied by scale
  * (the reference's grid3D::layer_wise_DFT2/IDFT2 scale argument,
@@ -237,6 +258,9 @@ typedef struct pfs_server_stats_t {
     size_t device_bytes;           /* tables + work buffers held            */
     size_t slot_bytes;             /* work-buffer bytes per slot            */
     uint64_t graphs;               /* batch graphs captured (replayed after) */
+    uint64_t tracked;              /* tickets the server still holds: not yet
+                                      consumed by pfs_server_wait or
+                                      pfs_server_collect                    */
 } pfs_server_stats_t;
 
 int pfs_server_create(pfs_context* ctx, const int* bandwidths, int nbw, int slots, int max_batch,
@@ -245,11 +269,15 @@ int pfs_server_create(pfs_context* ctx, const int* bandwidths, int nbw, int slot
 int pfs_server_destroy(pfs_server* srv);
 int pfs_server_submit(pfs_server* srv, int kind, int bandwidth, const void* in_dev, void* out_dev, uint64_t* ticket);
 /* Blocks until the request is done (timeout_s < 0: no limit); latency_s =
- * completion - submission, host clock.  Returns the request's status. */
+ * completion - submission, host clock.  Returns the request's status, and
+ * then forgets the ticket.  PFS_ERR_TIMEOUT: the request is still queued or
+ * running -- its buffers are still in use and the ticket stays valid (wait
+ * again, or take it from pfs_server_collect). */
 int pfs_server_wait(pfs_server* srv, uint64_t ticket, double timeout_s, double* latency_s);
 /* Up to cap requests finished since the last call, oldest first. */
 int pfs_server_collect(pfs_server* srv, uint64_t* tickets, double* latency_s, int* status, int cap, int* n_out);
-/* Blocks until nothing is waiting or running. */
+/* Blocks until nothing is waiting or running (PFS_ERR_TIMEOUT if the limit
+ * passes first). */
 int pfs_server_drain(pfs_server* srv, double timeout_s);
 int pfs_server_stats(pfs_server* srv, pfs_server_stats_t* out);
 /* Signals per batch the server admits at this bandwidth. */

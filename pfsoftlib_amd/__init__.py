@@ -36,13 +36,26 @@ PFS_REQ_INVERSE = 1
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("signals", ctypes.c_uint64),
                 ("pending", ctypes.c_uint64), ("max_inflight_batches", ctypes.c_int),
-                ("device_bytes", ctypes.c_size_t), ("slot_bytes", ctypes.c_size_t), ("graphs", ctypes.c_uint64)]
+                ("device_bytes", ctypes.c_size_t), ("slot_bytes", ctypes.c_size_t), ("graphs", ctypes.c_uint64),
+                ("tracked", ctypes.c_uint64)]
+
+
+PFS_ERR_ARG = 1
+PFS_ERR_TIMEOUT = 7
 
 
 class PfsError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("pfsoft_amd error %d: %s" % (code, msg))
         self.code = code
+
+
+class PfsArgumentError(PfsError, ValueError):
+    """A buffer argument rejected on the host before any device work (wrong
+    dtype, layout, device or size)."""
+
+    def __init__(self, msg):
+        PfsError.__init__(self, PFS_ERR_ARG, msg)
 
 
 def lib():
@@ -77,6 +90,9 @@ def lib():
             "pfs_layer_wise_DFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
             "pfs_layer_wise_IDFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
             "pfs_layer_wise_dft2_dev": (ip, [vp, ip, ip, vp, ip, ctypes.c_double, vp]),
+            "pfs_layer_wise_dft2": (ip, [ip, ip, ip, dp, ip, ctypes.c_double, ctypes.c_double, ip]),
+            "uzl_fftw_layer_wise_DFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
+            "uzl_fftw_layer_wise_IDFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
             "pfs_quadrature_weights": (ip, [vp, ip, dp]),
             "pfs_wigner_d_matrix": (ip, [vp, ip, ip, ip, ip, dp]),
             "pfs_stage_fft_fwd_dev": (ip, [vp, ip, vp, vp, vp]),
@@ -142,6 +158,29 @@ def _stream_handle(stream):
 
 def _devptr(t):
     return t if isinstance(t, int) else int(t.data_ptr())
+
+
+def _devbuf(t, what, dtype, n_complex, device):
+    """Device address of a buffer argument.  A torch tensor is checked before
+    its address goes to the library: complex dtype (complex128 for the FP64
+    path, complex64 for the mixed-precision one), contiguous, on this
+    context's device, and at least n_complex elements -- an undersized or
+    mistyped buffer would otherwise make the kernels read and write out of
+    its bounds.  A raw integer address is passed as given: its owner vouches
+    for it."""
+    if isinstance(t, int):
+        return t
+    import torch
+    want = torch.complex128 if dtype == "c128" else torch.complex64
+    if t.dtype != want:
+        raise PfsArgumentError("%s: expected a %s tensor, got %s" % (what, want, t.dtype))
+    if not t.is_contiguous():
+        raise PfsArgumentError("%s: tensor must be contiguous" % what)
+    if t.device.type != "cuda" or t.device.index != device:
+        raise PfsArgumentError("%s: tensor must live on cuda:%d, got %s" % (what, device, t.device))
+    if t.numel() < n_complex:
+        raise PfsArgumentError("%s: needs at least %d elements, got %d" % (what, n_complex, t.numel()))
+    return int(t.data_ptr())
 
 
 class Context:
@@ -223,20 +262,31 @@ class Context:
         return g
 
     # ---- device-buffer transforms (torch tensors or raw pointers) -------
+    # Tensor arguments are checked on the host (_devbuf) before any device work.
+    def _grid(self, t, what, B, dtype="c128", nsig=1):
+        return _devbuf(t, what, dtype, nsig * (2 * B) ** 3 if B > 0 else 0, self.device)
+
+    def _coef(self, t, what, B, dtype="c128", nsig=1):
+        return _devbuf(t, what, dtype, nsig * coeff_count(B) if B > 0 else 0, self.device)
+
     def dsoft_dev(self, B, grid_dev, coeffs_dev, stream=None):
-        _check(lib().pfs_dsoft_f64_dev(self._h, B, _devptr(grid_dev), _devptr(coeffs_dev), _stream_handle(stream)))
+        _check(lib().pfs_dsoft_f64_dev(self._h, B, self._grid(grid_dev, "grid", B),
+                                       self._coef(coeffs_dev, "coeffs", B), _stream_handle(stream)))
 
     def idsoft_dev(self, B, coeffs_dev, grid_dev, stream=None):
-        _check(lib().pfs_idsoft_f64_dev(self._h, B, _devptr(coeffs_dev), _devptr(grid_dev), _stream_handle(stream)))
+        _check(lib().pfs_idsoft_f64_dev(self._h, B, self._coef(coeffs_dev, "coeffs", B),
+                                        self._grid(grid_dev, "grid", B), _stream_handle(stream)))
 
     def dsoft_batch_f32(self, B, nsig, grids_dev, coeffs_dev, stream=None):
         """nsig complex64 grids -> nsig packed complex64 coefficient sets (mixed precision)."""
-        _check(lib().pfs_dsoft_batch_f32_dev(self._h, B, nsig, _devptr(grids_dev), _devptr(coeffs_dev),
-                                             _stream_handle(stream)))
+        n = max(int(nsig), 0)
+        _check(lib().pfs_dsoft_batch_f32_dev(self._h, B, nsig, self._grid(grids_dev, "grids", B, "c64", n),
+                                             self._coef(coeffs_dev, "coeffs", B, "c64", n), _stream_handle(stream)))
 
     def idsoft_batch_f32(self, B, nsig, coeffs_dev, grids_dev, stream=None):
-        _check(lib().pfs_idsoft_batch_f32_dev(self._h, B, nsig, _devptr(coeffs_dev), _devptr(grids_dev),
-                                              _stream_handle(stream)))
+        n = max(int(nsig), 0)
+        _check(lib().pfs_idsoft_batch_f32_dev(self._h, B, nsig, self._coef(coeffs_dev, "coeffs", B, "c64", n),
+                                              self._grid(grids_dev, "grids", B, "c64", n), _stream_handle(stream)))
 
     def autotune_batch(self, B, max_signals):
         """Times the batched contraction shapes for 1, 2, 4, ... max_signals signals."""
@@ -249,19 +299,24 @@ class Context:
         return w.value, s.value
 
     def stage_fft_fwd(self, B, grid_dev, pencils_dev, stream=None):
-        _check(lib().pfs_stage_fft_fwd_dev(self._h, B, _devptr(grid_dev), _devptr(pencils_dev), _stream_handle(stream)))
+        _check(lib().pfs_stage_fft_fwd_dev(self._h, B, self._grid(grid_dev, "grid", B),
+                                           self._grid(pencils_dev, "pencils", B), _stream_handle(stream)))
 
     def stage_dwt_fwd(self, B, pencils_dev, coeffs_dev, stream=None):
-        _check(lib().pfs_stage_dwt_fwd_dev(self._h, B, _devptr(pencils_dev), _devptr(coeffs_dev), _stream_handle(stream)))
+        _check(lib().pfs_stage_dwt_fwd_dev(self._h, B, self._grid(pencils_dev, "pencils", B),
+                                           self._coef(coeffs_dev, "coeffs", B), _stream_handle(stream)))
 
     def stage_dwt_inv(self, B, coeffs_dev, pencils_dev, stream=None):
-        _check(lib().pfs_stage_dwt_inv_dev(self._h, B, _devptr(coeffs_dev), _devptr(pencils_dev), _stream_handle(stream)))
+        _check(lib().pfs_stage_dwt_inv_dev(self._h, B, self._coef(coeffs_dev, "coeffs", B),
+                                           self._grid(pencils_dev, "pencils", B), _stream_handle(stream)))
 
     def stage_fft_inv(self, B, pencils_dev, grid_dev, stream=None):
-        _check(lib().pfs_stage_fft_inv_dev(self._h, B, _devptr(pencils_dev), _devptr(grid_dev), _stream_handle(stream)))
+        _check(lib().pfs_stage_fft_inv_dev(self._h, B, self._grid(pencils_dev, "pencils", B),
+                                           self._grid(grid_dev, "grid", B), _stream_handle(stream)))
 
     def layer_wise_dft2_dev(self, n, lays, arr_dev, sign, scale=1.0, stream=None):
-        _check(lib().pfs_layer_wise_dft2_dev(self._h, n, lays, _devptr(arr_dev), sign, scale, _stream_handle(stream)))
+        ptr = _devbuf(arr_dev, "arr", "c128", max(int(n), 0) ** 2 * max(int(lays), 0), self.device)
+        _check(lib().pfs_layer_wise_dft2_dev(self._h, n, lays, ptr, sign, scale, _stream_handle(stream)))
 
     def synchronize(self, stream=None):
         _check(lib().pfs_synchronize(self._h, _stream_handle(stream)))
@@ -315,9 +370,10 @@ def _warn(msg):
 class DSOFTFourierCoefficients:
     """Coefficients f^l_{M,M'} for 0 <= l < bandwidth, |M|,|M'| <= l, packed by
     degree (dsoft_fourier_coefficients.cpp:48-112).  fc(l, M, Mp) is the
-    element; negative orders count from the end of the degree block."""
+    element; negative orders count from the end of the degree block.  With no
+    argument the container is empty (bandwidth 0, :31-35)."""
 
-    def __init__(self, bandwidth):
+    def __init__(self, bandwidth=0):
         self.bandwidth = int(bandwidth)
         self.mem = np.zeros(coeff_count(self.bandwidth), dtype=np.complex128)
 
@@ -336,13 +392,32 @@ class DSOFTFourierCoefficients:
 
 class grid3D:
     """rows x cols x lays complex128 grid, element (row, col, lay) at
-    rows*cols*lay + cols*col + row (grid3D_cx.hpp:293-303)."""
+    rows*cols*lay + cols*col + row (grid3D_cx.hpp:293-303).  New grids hold
+    `initial` (0 by default; a real initial value has imaginary part 0,
+    grid3D_cx.hpp:61-70)."""
 
-    def __init__(self, rows, cols=None, lays=None):
+    def __init__(self, rows=0, cols=None, lays=None, initial=0):
         self.rows = int(rows)
         self.cols = int(cols if cols is not None else rows)
         self.lays = int(lays if lays is not None else rows)
-        self.mem = np.zeros(self.rows * self.cols * self.lays, dtype=np.complex128)
+        self.mem = np.full(self.rows * self.cols * self.lays, complex(initial), dtype=np.complex128)
+
+    def layer_wise_DFT2(self, scale=1.0, threads=1):
+        """Unnormalised 2D DFT (sign -1) of every slice, in place, times
+        scale (grid3D_cx.hpp:309-366); on the device."""
+        self._dft2(-1, scale, threads)
+
+    def layer_wise_IDFT2(self, scale=1.0, threads=1):
+        """Unnormalised inverse 2D DFT (sign +1), in place, times scale
+        (grid3D_cx.hpp:370-427)."""
+        self._dft2(+1, scale, threads)
+
+    def _dft2(self, sign, scale, threads):
+        if self.mem.size == 0:
+            return
+        sc = complex(scale)
+        _check(lib().pfs_layer_wise_dft2(self.cols, self.rows, self.lays, _dptr(self.mem.view(np.float64)), sign,
+                                         sc.real, sc.imag, threads))
 
     def __call__(self, row, col, lay):
         return self.mem[self.rows * self.cols * lay + self.cols * col + row]
@@ -389,18 +464,30 @@ class Server:
         self._keep = {}  # ticket -> buffers, kept alive until the request is consumed
 
     def submit(self, kind, B, inp, out):
+        # complex64 buffers: forward grid -> coefficients, inverse the reverse
+        if kind in (PFS_REQ_FORWARD, PFS_REQ_INVERSE) and B > 0:
+            ng, nc = (2 * B) ** 3, coeff_count(B)
+            n_in, n_out = (ng, nc) if kind == PFS_REQ_FORWARD else (nc, ng)
+            pin = _devbuf(inp, "in", "c64", n_in, self.ctx.device)
+            pout = _devbuf(out, "out", "c64", n_out, self.ctx.device)
+        else:  # the library rejects the kind or bandwidth
+            pin, pout = _devptr(inp), _devptr(out)
         t = ctypes.c_uint64()
-        _check(lib().pfs_server_submit(self._h, kind, B, _devptr(inp), _devptr(out), ctypes.byref(t)))
+        _check(lib().pfs_server_submit(self._h, kind, B, pin, pout, ctypes.byref(t)))
         self._keep[t.value] = (inp, out)
         return t.value
 
     def wait(self, ticket, timeout=-1.0):
-        """Blocks until the request is done; returns its latency in seconds."""
+        """Blocks until the request is done; returns its latency in seconds.
+        On a timeout (PfsError with code PFS_ERR_TIMEOUT) the request is still
+        queued or running: its buffers stay referenced and the ticket stays
+        valid for another wait or for collect()."""
         lat = ctypes.c_double()
-        try:
-            _check(lib().pfs_server_wait(self._h, ticket, timeout, ctypes.byref(lat)))
-        finally:
+        rc = lib().pfs_server_wait(self._h, ticket, timeout, ctypes.byref(lat))
+        if rc != PFS_ERR_TIMEOUT:
+            # terminal: done or failed on the device -- the buffers are free
             self._keep.pop(ticket, None)
+        _check(rc)
         return lat.value
 
     def collect(self, cap=4096):
@@ -441,6 +528,6 @@ class Server:
             pass
 
 
-__all__ = ["Context", "PfsError", "DSOFT", "IDSOFT", "DSOFTFourierCoefficients", "grid3D",
+__all__ = ["Context", "PfsError", "PfsArgumentError", "PFS_ERR_TIMEOUT", "DSOFT", "IDSOFT", "DSOFTFourierCoefficients", "grid3D",
            "coeff_count", "coeff_index", "coeff_offset", "seeded_coeffs", "layer_wise_DFT2_grid3D", "lib",
            "Server", "PFS_REQ_FORWARD", "PFS_REQ_INVERSE"]

@@ -48,7 +48,9 @@ int fail(int code, const char* fmt, ...)
                         "%s failed: %s", #expr, hipGetErrorString(e_));                    \
     } while (0)
 
-constexpr int MAX_BANDWIDTH = 512;
+// Largest bandwidth the transforms accept: every kernel configuration up to
+// 2B = 512 runs in the GPU tests; 2B = 1024 has never run, so it is refused.
+constexpr int MAX_BANDWIDTH = 256;
 
 __global__ void k_plan_nodes(int B, double* cosb, double* sinh, double* cosh, double* wts, double2* tw, float2* tw32)
 {
@@ -757,7 +759,20 @@ int idsoft_dev(pfs_context* c, int B, const double2* coeffs, double2* grid, hipS
     return fft_inv(c, *p, G, W, grid, st);
 }
 
-int dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double scale, hipStream_t st)
+// x *= (sr + i si) over n complex values (the reference's serial scale loop
+// after a layer-wise DFT2 with a complex scale, grid3D_cx.hpp:342-356).
+__global__ void k_scale_cx(double2* x, long n, double sr, double si)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double2 v = x[i];
+        x[i] = make_double2(v.x * sr - v.y * si, v.x * si + v.y * sr);
+    }
+}
+
+// scale_im != 0: the transform runs unscaled, then one pass multiplies by the
+// complex scale; a real scale is folded into the transform's second pass.
+int dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double scale, hipStream_t st,
+             double scale_im = 0.0)
 {
     if (!c || !arr_dev || n < 2 || (n & 1) || lays < 1 || (sign != 1 && sign != -1))
         return fail(PFS_ERR_ARG, "pfs_layer_wise_dft2_dev: bad arguments (n must be even and >= 2)");
@@ -779,7 +794,13 @@ int dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double sc
         LineMap m2 = map_cols_seam(n);
         m2.nu = kc;
         HIP_TRY(launch_fft_lines_f64(a + k0 * n2, W, m1, n, sign, 1.0, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
-        HIP_TRY(launch_fft_lines_f64(W, a + k0 * n2, m2, n, sign, scale, p->t.tw, LINES_ROW_IN_TILE_OUT, 1, st));
+        HIP_TRY(launch_fft_lines_f64(W, a + k0 * n2, m2, n, sign, scale_im != 0.0 ? 1.0 : scale, p->t.tw,
+                                     LINES_ROW_IN_TILE_OUT, 1, st));
+    }
+    if (scale_im != 0.0) {
+        const long tot = n2 * lays;
+        const int blocks = (int)std::min<long>((tot + 255) / 256, 8192);
+        HIP_TRY(pfs::launch(k_scale_cx, dim3(blocks), dim3(256), 0, st, a, tot, scale, scale_im));
     }
     return PFS_OK;
 }
@@ -1060,7 +1081,7 @@ int idsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* coeffs, floa
 std::mutex g_seam_mu;
 pfs_context* g_seam_ctx = nullptr;
 
-int seam_dft2(int cols, int rows, int lays, double* arr, int sign)
+int seam_dft2(int cols, int rows, int lays, double* arr, int sign, double scale_re = 1.0, double scale_im = 0.0)
 {
     g_last_error.clear();  // the seam returns void: callers read pfs_last_error()
     if (!arr || cols < 1 || rows < 1 || lays < 0) return fail(PFS_ERR_ARG, "layer_wise DFT2: bad arguments");
@@ -1081,7 +1102,7 @@ int seam_dft2(int cols, int rows, int lays, double* arr, int sign)
     HIP_TRY(hipSetDevice(c->device));
     rc = copy_h2d(c, c->io_grid, arr, bytes, c->stream);
     if (rc) return rc;
-    rc = dft2_dev(c, cols, lays, c->io_grid, sign, 1.0, c->stream);
+    rc = dft2_dev(c, cols, lays, c->io_grid, sign, scale_re, c->stream, scale_im);
     if (rc) return rc;
     return copy_d2h(c, arr, c->io_grid, bytes, c->stream);
 }
@@ -1259,6 +1280,24 @@ void pfs_layer_wise_IDFT2_grid3D(int cols, int rows, int lays, double* arr, int 
     (void)threads;
     int rc = seam_dft2(cols, rows, lays, arr, +1);
     if (rc) fprintf(stderr, "** [pfsoft_amd error] layer_wise_IDFT2_grid3D: %s **\n", pfs_last_error());
+}
+
+void uzl_fftw_layer_wise_DFT2_grid3D(int cols, int rows, int lays, double* arr, int threads)
+{
+    pfs_layer_wise_DFT2_grid3D(cols, rows, lays, arr, threads);
+}
+
+void uzl_fftw_layer_wise_IDFT2_grid3D(int cols, int rows, int lays, double* arr, int threads)
+{
+    pfs_layer_wise_IDFT2_grid3D(cols, rows, lays, arr, threads);
+}
+
+int pfs_layer_wise_dft2(int cols, int rows, int lays, double* arr, int sign, double scale_re, double scale_im,
+                        int threads)
+{
+    (void)threads;
+    if (sign != 1 && sign != -1) return fail(PFS_ERR_ARG, "pfs_layer_wise_dft2: sign must be -1 or +1");
+    return seam_dft2(cols, rows, lays, arr, sign, scale_re, scale_im);
 }
 
 int pfs_layer_wise_dft2_dev(pfs_context* c, int n, int lays, void* arr_dev, int sign, double scale, void* stream)

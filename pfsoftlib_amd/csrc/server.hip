@@ -401,10 +401,15 @@ int pfs_server_wait(pfs_server* s, uint64_t ticket, double timeout_s, double* la
     if (it == s->reqs.end()) return set_error(PFS_ERR_ARG, "pfs_server_wait: unknown or already collected ticket");
     const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s < 0 ? 1e9 : timeout_s);
     if (!s->cv_done.wait_until(lk, until, [&] { return s->reqs[ticket].status >= 0; }))
-        return set_error(PFS_ERR_ARG, "pfs_server_wait: timed out");
+        return set_error(PFS_ERR_TIMEOUT, "pfs_server_wait: timed out; the request is still queued or running");
     const Request& r = s->reqs[ticket];
     if (latency_s) *latency_s = r.t_done - r.t_submit;
     const int st = r.status;
+    // the ticket is consumed: forget the request and its entry in the list
+    // pfs_server_collect would report
+    s->reqs.erase(ticket);
+    auto f = std::find(s->finished.begin(), s->finished.end(), ticket);
+    if (f != s->finished.end()) s->finished.erase(f);
     if (st) {
         const std::string msg = "request failed: " + s->error;
         lk.unlock();
@@ -443,7 +448,7 @@ int pfs_server_drain(pfs_server* s, double timeout_s)
             if (sl.busy) return false;
         return true;
     };
-    if (!s->cv_done.wait_until(lk, until, idle)) return set_error(PFS_ERR_ARG, "pfs_server_drain: timed out");
+    if (!s->cv_done.wait_until(lk, until, idle)) return set_error(PFS_ERR_TIMEOUT, "pfs_server_drain: timed out");
     return PFS_OK;
 }
 
@@ -459,6 +464,7 @@ int pfs_server_stats(pfs_server* s, pfs_server_stats_t* out)
     out->device_bytes = s->device_bytes;
     out->slot_bytes = s->slot_bytes;
     out->graphs = s->n_graphs;
+    out->tracked = s->reqs.size();
     return PFS_OK;
 }
 

@@ -17,7 +17,7 @@ def declared_functions():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        for m in re.finditer(r"\b(pfs_\w+)\s*\(", src):
+        for m in re.finditer(r"\b((?:pfs|uzl)_\w+)\s*\(", src):
             names.add(m.group(1))
     return sorted(names)
 
@@ -25,7 +25,8 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for n in ["pfs_dsoft_f64", "pfs_idsoft_f64", "pfs_dsoft_f64_dev", "pfs_idsoft_f64_dev",
-              "pfs_layer_wise_DFT2_grid3D", "pfs_layer_wise_IDFT2_grid3D", "pfs_ctx_create", "pfs_last_error"]:
+              "pfs_layer_wise_DFT2_grid3D", "pfs_layer_wise_IDFT2_grid3D", "pfs_ctx_create", "pfs_last_error",
+              "uzl_fftw_layer_wise_DFT2_grid3D", "uzl_fftw_layer_wise_IDFT2_grid3D", "pfs_layer_wise_dft2"]:
         assert n in names
 
 
