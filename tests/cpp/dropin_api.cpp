@@ -68,11 +68,14 @@ int main()
         check(e.rows == 0 && e.mem == nullptr, "grid3D() is empty");
         grid3D<complex<double> > a(4, complex<double>(1, -2));
         check(a(3, 2, 1).re == 1 && a(3, 2, 1).im == -2, "grid3D(rcl, complex initial)");
+        // (the index rows*cols*lay + cols*col + row stays inside a non-square
+        // layer only for cols*col + row < rows*cols, grid3D_cx.hpp:295)
         grid3D<complex<double> > b(2, 3, 4, 7.0);
-        check(b.rows == 2 && b.cols == 3 && b.lays == 4 && b(1, 2, 3).re == 7 && b(1, 2, 3).im == 0,
+        check(b.rows == 2 && b.cols == 3 && b.lays == 4 && b(1, 1, 3).re == 7 && b(1, 1, 3).im == 0 &&
+                  b.mem[23].re == 7,
               "grid3D(rows, cols, lays, real initial)");
         grid3D<complex<double> > c(2, 3, 4);
-        check(c(1, 2, 3).re == 0 && c(1, 2, 3).im == 0, "grid3D(rows, cols, lays) is zero");
+        check(c(1, 1, 3).re == 0 && c(1, 1, 3).im == 0, "grid3D(rows, cols, lays) is zero");
         grid3D<complex<double> > d(std::move(a));
         check(d(3, 2, 1).im == -2 && a.mem == nullptr, "grid3D move keeps the values");
         DSOFTFourierCoefficients none;
