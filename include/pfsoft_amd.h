@@ -94,6 +94,21 @@ int pfs_idsoft_f64(pfs_context* ctx, int bandwidth, const double* coeffs, double
 int pfs_dsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* grid_dev, void* coeffs_dev, void* stream);
 int pfs_idsoft_f64_dev(pfs_context* ctx, int bandwidth, const void* coeffs_dev, void* grid_dev, void* stream);
 
+/* Forward transform on device buffers that allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as
+ * pfs_dsoft_f64_dev (bit for bit), but its two work buffers come from a
+ * file-scope device array of 64 MiB per GPU instead of the context, so:
+ *   - bandwidth <= 64 (2B <= 128); larger ones return PFS_ERR_UNSUPPORTED;
+ *   - the bandwidth's plan must already be on the context
+ *     (pfs_plan_warm(ctx, bandwidth)), else PFS_ERR_ARG;
+ *   - every call on one GPU shares that GPU's array, whatever its context:
+ *     calls that overlap in time on one GPU race;
+ *   - the calls launch kernels only (no allocation, no timing events, no
+ *     memset), so they can be captured into a HIP graph. */
+int pfs_dsoft_f64_dev_noalloc(pfs_context* ctx, int bandwidth, const void* grid_dev, void* coeffs_dev, void* stream);
+
 /* Batched mixed-precision transforms of nsig signals of one bandwidth
  * (no counterpart in the reference, which transforms one complex128 grid per
  * call).  grids: nsig consecutive N^3 complex64 grids (reference layout);

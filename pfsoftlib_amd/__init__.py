@@ -87,6 +87,7 @@ def lib():
             "pfs_idsoft_f64": (ip, [vp, ip, dp, dp]),
             "pfs_dsoft_f64_dev": (ip, [vp, ip, vp, vp, vp]),
             "pfs_idsoft_f64_dev": (ip, [vp, ip, vp, vp, vp]),
+            "pfs_dsoft_f64_dev_noalloc": (ip, [vp, ip, vp, vp, vp]),
             "pfs_layer_wise_DFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
             "pfs_layer_wise_IDFT2_grid3D": (None, [ip, ip, ip, dp, ip]),
             "pfs_layer_wise_dft2_dev": (ip, [vp, ip, ip, vp, ip, ctypes.c_double, vp]),
@@ -276,6 +277,13 @@ class Context:
     def idsoft_dev(self, B, coeffs_dev, grid_dev, stream=None):
         _check(lib().pfs_idsoft_f64_dev(self._h, B, self._coef(coeffs_dev, "coeffs", B),
                                         self._grid(grid_dev, "grid", B), _stream_handle(stream)))
+
+    def dsoft_dev_noalloc(self, B, grid_dev, coeffs_dev, stream=None):
+        """dsoft_dev without any device allocation (work buffers in the
+        library's 64 MiB per-GPU scratch array): B <= 64, plan warmed first
+        (warm(B)); concurrent calls on one GPU race."""
+        _check(lib().pfs_dsoft_f64_dev_noalloc(self._h, B, self._grid(grid_dev, "grid", B),
+                                               self._coef(coeffs_dev, "coeffs", B), _stream_handle(stream)))
 
     def dsoft_batch_f32(self, B, nsig, grids_dev, coeffs_dev, stream=None):
         """nsig complex64 grids -> nsig packed complex64 coefficient sets (mixed precision)."""

@@ -1134,6 +1134,9 @@ int pfs_ctx_create(int device, size_t hbm_budget_bytes, pfs_context** out)
     }
     int rc = mfma_selftest(c->stream);
     if (!rc) rc = mfma_selftest_f32(c->stream);
+    // the device address of this device's allocation-free scratch array (once
+    // per device; a failure is reported by the entry point that needs it)
+    if (!rc) pfs::noalloc_setup(device);
     if (rc) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1682,6 +1685,29 @@ int set_error(int code, const char* msg)
 }
 
 const char* error_text() { return g_last_error.c_str(); }
+
+int dsoft_prepared(pfs_context* c, int B, const double2* grid, double2* coeffs, double2* W, double2* G,
+                   hipStream_t st)
+{
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_bandwidth(B);
+    if (rc) return rc;
+    auto it = c->plans.find(B);  // a lookup only: building a plan allocates
+    if (it == c->plans.end())
+        return fail(PFS_ERR_ARG, "bandwidth %d: plan not prepared on this context (call pfs_plan_warm first)", B);
+    HIP_TRY(hipSetDevice(c->device));
+    // the same launches as dsoft_dev, without timing events (creating one allocates)
+    const bool timing = c->timing;
+    c->timing = false;
+    rc = fft_fwd(c, it->second, grid, W, G, st);
+    if (!rc) {
+        const hipError_t e = launch_dwt_fwd_f64(dwt_tables(c, it->second, true), G, coeffs, fwd_norm(B), st,
+                                                (int)c->dwt_variant);
+        if (e != hipSuccess) rc = fail(PFS_ERR_DEVICE, "forward DWT launch failed: %s", hipGetErrorString(e));
+    }
+    c->timing = timing;
+    return rc;
+}
 
 int context_device(const pfs_context* c) { return c->device; }
 

@@ -244,6 +244,14 @@ int set_error(int code, const char* msg);
 const char* error_text();
 // Device ordinal of a context.
 int context_device(const pfs_context* c);
+// Forward FP64 transform with caller-provided work buffers W and G (N^3
+// complex128 each) for a bandwidth whose plan the context already holds;
+// launches kernels only: no allocation, no timing events.  Takes the lock.
+int dsoft_prepared(pfs_context* c, int B, const double2* grid, double2* coeffs, double2* W, double2* G,
+                   hipStream_t st);
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (which must be current), once per device (noalloc.hip).
+void noalloc_setup(int device);
 // Whether the batched-shape autotuner has covered batches of up to `upto`
 // signals at bandwidth B on this context.
 bool batch_is_tuned(pfs_context* c, int B, int upto);
