@@ -216,8 +216,10 @@ def batch_dwt_flops(B):
     return f
 
 
-def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
-    """The batched mixed-precision round trip over nsig complex64 signals.
+def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0, prec=0):
+    """The batched mixed-precision round trip over nsig complex64 signals,
+    with the Wigner contraction in FP32 MFMA (prec 0) or in split-bf16 x3 / x6
+    products on bf16 MFMA (prec 3 / 6, context tune key batch_precision).
 
     Every signal's result is checked against the plain FP64 GPU path (which
     the parity tests pin to the CPU oracle at <= 1e-11) on the same input:
@@ -230,6 +232,7 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
     grids = torch.empty((nsig, N ** 3), dtype=torch.complex64, device="cuda")
     back = torch.empty_like(c32)
     ctx.warm(B)
+    ctx.tune("batch_precision", prec)
 
     def step():
         ctx.idsoft_batch_f32(B, nsig, c32, grids, stream=stream)
@@ -249,6 +252,7 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.timing(False)
+    ctx.tune("batch_precision", 0)
     lat = sorted(ev[i][0].elapsed_time(ev[i][1]) for i in range(steps))
 
     kernels = {}
@@ -291,8 +295,11 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0):
         ref = c32[i].to(torch.complex128)
         e_rt.append((nrm(back[i].to(torch.complex128) - ref) / nrm(ref)).item())
     elapsed = t1 - t0
+    contraction = {0: "fp32 MFMA", 3: "split-bf16 x3 on bf16 MFMA (fp32 accumulation)",
+                   6: "split-bf16 x6 on bf16 MFMA (fp32 accumulation)"}[prec]
     return {
-        "bandwidth": B, "signals": nsig, "precision": "fp32 FFT + fp32 MFMA DWT, fp64 tables/weights/twiddles",
+        "bandwidth": B, "signals": nsig,
+        "precision": "fp32 FFT + %s DWT, fp64 tables/weights/twiddles" % contraction,
         "pairs_per_s": nsig * steps / elapsed, "transforms_per_s": 2 * nsig * steps / elapsed,
         "ms_per_batch_roundtrip": elapsed / steps * 1e3, "steps": steps, "warmup": warmup,
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
@@ -668,6 +675,13 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_c4:
         out["batched_mixed_512xB64"] = batched_leg(torch, P, ctx, 64, 512, args.batch_steps, 1, args.seed, args.batch_check)
+        # the split-bf16 contractions on the same workload, with their error against FP64
+        for prec in (3, 6):
+            r = batched_leg(torch, P, ctx, 64, 512, args.batch_steps, 1, args.seed, args.batch_check, prec)
+            out["batched_512xB64_bf16x%d" % prec] = {
+                k: r[k] for k in ("precision", "pairs_per_s", "ms_per_batch_roundtrip", "rel_l2_vs_fp64")}
+            out["batched_512xB64_bf16x%d" % prec]["kernels"] = {
+                k: v for k, v in r["kernels"].items() if k.startswith("batch_dwt_")}
     if args.serve_seconds > 0:
         out["serving"] = serving_leg(torch, P, ctx, args.serve_seconds, args.seed, int(args.serve_budget_gb * 2 ** 30),
                                      args.serve_slots, args.serve_max_batch, rate=args.serve_rate or None)

@@ -210,7 +210,15 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *                       chunks keep the row-pass output in the Infinity
  *                       Cache at the price of more, smaller launches.
  *   "batch_chunk_signals" signals per sub-batch of the batched path (work
- *                       buffers hold this many complex64 grids; default 64). */
+ *                       buffers hold this many complex64 grids; default 64).
+ *   "batch_precision"   products of the batched Wigner contraction: 0 FP32
+ *                       MFMA (default); 3 split-bf16 x3 (h h' + h m' + m h'
+ *                       on bf16 MFMA, FP32 accumulation); 6 split-bf16 x6
+ *                       (adds h l' + m m' + l h').  The bf16 forms keep the
+ *                       bf16 terms of the Wigner tables resident (6 bytes per
+ *                       table value and table) and need the context's bf16
+ *                       MFMA self-test to have passed (else
+ *                       PFS_ERR_UNSUPPORTED). */
 int pfs_tune_set(pfs_context* ctx, const char* key, long value);
 int pfs_tune_get(pfs_context* ctx, const char* key, long* value);
 

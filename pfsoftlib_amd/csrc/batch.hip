@@ -18,11 +18,64 @@
 // src/fn_dsoft.cpp:112-283, src/fn_idsoft.cpp:100-283).
 #include "pfs_internal.h"
 
+#include <algorithm>
+
 namespace pfs {
 
 typedef float v2f __attribute__((ext_vector_type(2)));
+typedef short v4s __attribute__((ext_vector_type(4)));
 
 int g_mfma_f32_layout = 0;
+bool g_mfma_bf16_ok = false;
+
+// ---- split-BF16 products ------------------------------------------------------
+// A value x is split into bf16 terms h + m (+ l): h = bf16_rne(x), m =
+// bf16_rne(x - h), l = bf16_rne(x - h - m); each remainder is exact in FP32,
+// and each term keeps 8 more significant bits.  PREC 3 sums the products
+// h h' + h m' + m h' (the terms of order 2^-16 dropped), PREC 6 adds
+// h l' + m m' + l h' (order 2^-24 dropped), on v_mfma_f32_16x16x16_bf16 with
+// FP32 accumulation.  PREC 0 is the plain FP32 MFMA path.
+
+// bf16 bit pattern of round-to-nearest-even(x), in the high 16 bits (finite x).
+__device__ __forceinline__ unsigned bf16_rne_bits(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+}
+
+// Bf16 terms of four values, packed as the K-ordered operand of one
+// 16x16x16 bf16 MFMA: element u of part p is term p of x[u].
+template <int PARTS>
+__device__ __forceinline__ void split4(const float (&x)[4], v4s (&out)[PARTS])
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        float r = x[u];
+#pragma unroll
+        for (int pp = 0; pp < PARTS; ++pp) {
+            const unsigned hb = bf16_rne_bits(r);
+            out[pp][u] = (short)(hb >> 16);
+            r -= __uint_as_float(hb);
+        }
+    }
+}
+
+// acc += A B over the split terms (A: ap parts, B: bp parts), smallest first.
+template <int PREC>
+__device__ __forceinline__ v4f mfma_split(const v4s* ap, const v4s* bp, v4f acc)
+{
+    if constexpr (PREC == 6) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[0], bp[2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[1], bp[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[2], bp[0], acc, 0, 0, 0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[0], bp[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[1], bp[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[0], bp[0], acc, 0, 0, 0);
+    return acc;
+}
+
+constexpr int split_parts(int prec) { return prec == 6 ? 3 : 2; }
 
 namespace {
 
@@ -134,9 +187,10 @@ __device__ __forceinline__ BlockWork block_work(int nsg)
 // step t contracts betas 16 kb + 4 kk + t over the lane groups kk.  The loads
 // of block kb + 1 are issued before the MFMAs of block kb (two register
 // stages), so memory latency overlaps the contraction.
-template <int NN, int SPW, int NC>
+template <int NN, int SPW, int NC, int PREC = 0>
 struct FwdStage {
-    float4 a[NC];
+    float4 a[PREC ? 1 : NC];                      // FP32 table operands
+    v4s as[PREC ? NC : 1][split_parts(PREC)];     // split table operands
     float4 g[SPW];  // two consecutive complex values of this lane's case pencil
     float4 w;
 };
@@ -153,13 +207,27 @@ __device__ __forceinline__ float swap_pair(float v)
 // + 2(1-p) -- one 16-byte load per lane, 1 KB contiguous per wave (8 pencils
 // x 128 bytes).  The partners then swap halves so that the even lane holds the
 // real parts and the odd lane the imaginary parts of the four betas, in t order.
-template <int NN, int SPW, int NC>
-__device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC>& S, const float* const (&gp)[SPW], const float* tf,
-                                         const float* wl, int e0, int de, int kb)
+// Table tile parts of the split precisions: same element order as the FP32
+// tiles, one array of bf16 bit patterns per term.
+struct SplitTab {
+    const unsigned short* p[3];
+};
+
+template <int NN, int SPW, int NC, int PREC>
+__device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC, PREC>& S, const float* const (&gp)[SPW], const float* tf,
+                                         const SplitTab& ts, long toff, const float* wl, int e0, int de, int kb)
 {
     constexpr int NKB = NN / 16;
+    if constexpr (PREC == 0) {
 #pragma unroll
-    for (int ch = 0; ch < NC; ++ch) S.a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
+        for (int ch = 0; ch < NC; ++ch) S.a[ch] = *reinterpret_cast<const float4*>(tf + (ch * NKB + kb) * 256);
+    } else {
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch)
+#pragma unroll
+            for (int pp = 0; pp < split_parts(PREC); ++pp)
+                S.as[ch][pp] = *reinterpret_cast<const v4s*>(ts.p[pp] + toff + (ch * NKB + kb) * 256);
+    }
     const int e = e0 + kb * de;
     // each pencil value is read once: a non-temporal load does not allocate, so
     // the streamed pencils (8.6 GB for 64 grids at 2B = 256) do not evict the
@@ -172,8 +240,8 @@ __device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC>& S, const float* 
     S.w = *reinterpret_cast<const float4*>(wl + 16 * kb);
 }
 
-template <int NN, int SPW, int NC>
-__device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN, SPW, NC>& S, bool odd, bool flip)
+template <int NN, int SPW, int NC, int PREC>
+__device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN, SPW, NC, PREC>& S, bool odd, bool flip)
 {
     float b[SPW][4];
 #pragma unroll
@@ -192,21 +260,33 @@ __device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN,
         b[sp][2] = (flip ? d3 : d2) * S.w.z;
         b[sp][3] = (flip ? d2 : d3) * S.w.w;
     }
+    if constexpr (PREC == 0) {
 #pragma unroll
-    for (int ch = 0; ch < NC; ++ch)
+        for (int ch = 0; ch < NC; ++ch)
 #pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) {
-            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].x, b[sp][0], acc[ch][sp], 0, 0, 0);
-            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].y, b[sp][1], acc[ch][sp], 0, 0, 0);
-            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].z, b[sp][2], acc[ch][sp], 0, 0, 0);
-            acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].w, b[sp][3], acc[ch][sp], 0, 0, 0);
-        }
+            for (int sp = 0; sp < SPW; ++sp) {
+                acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].x, b[sp][0], acc[ch][sp], 0, 0, 0);
+                acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].y, b[sp][1], acc[ch][sp], 0, 0, 0);
+                acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].z, b[sp][2], acc[ch][sp], 0, 0, 0);
+                acc[ch][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[ch].w, b[sp][3], acc[ch][sp], 0, 0, 0);
+            }
+    } else {
+        // the four betas of an FP32 step group are the K = 16 of one bf16 MFMA
+        v4s bs[SPW][split_parts(PREC)];
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) split4<split_parts(PREC)>(b[sp], bs[sp]);
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch)
+#pragma unroll
+            for (int sp = 0; sp < SPW; ++sp) acc[ch][sp] = mfma_split<PREC>(S.as[ch], bs[sp], acc[ch][sp]);
+    }
 }
 
-template <int NN, int SPW, int NC>
-__device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const float* tf, const float* wl, int e0,
-                                         int de, float2* __restrict__ coeffs, const BCase& cb, int M, int R, int s0,
-                                         int nsig, long cdist, long cstride, float norm, int lane, int layout)
+template <int NN, int SPW, int NC, int PREC>
+__device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const float* tf, const SplitTab& ts, long toff,
+                                         const float* wl, int e0, int de, float2* __restrict__ coeffs, const BCase& cb,
+                                         int M, int R, int s0, int nsig, long cdist, long cstride, float norm, int lane,
+                                         int layout)
 {
     constexpr int NKB = NN / 16;
     v4f acc[NC][SPW];
@@ -221,11 +301,11 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
     // too little to cover a load's latency with one block in flight.
     constexpr int ST0 = NC <= 2 ? 4 : 2;
     constexpr int ST = ST0 < NKB ? ST0 : NKB;
-    FwdStage<NN, SPW, NC> S[ST];
+    FwdStage<NN, SPW, NC, PREC> S[ST];
 #pragma unroll
-    for (int st = 0; st + 1 < ST; ++st) fwd_load(S[st], gp, tf, wl, e0, de, st);
+    for (int st = 0; st + 1 < ST; ++st) fwd_load(S[st], gp, tf, ts, toff, wl, e0, de, st);
     if constexpr (ST == 1) {
-        fwd_load(S[0], gp, tf, wl, e0, de, 0);
+        fwd_load(S[0], gp, tf, ts, toff, wl, e0, de, 0);
         fwd_mfma(acc, S[0], odd, flip);
     } else {
         // The scheduling barriers keep each stage's loads ahead of the other
@@ -237,7 +317,7 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
                 // branch here would make the compiler drain every outstanding
                 // load before the next MFMAs
                 const int nk = kb + st + ST - 1;
-                fwd_load(S[(st + ST - 1) % ST], gp, tf, wl, e0, de, nk < NKB ? nk : NKB - 1);
+                fwd_load(S[(st + ST - 1) % ST], gp, tf, ts, toff, wl, e0, de, nk < NKB ? nk : NKB - 1);
                 __builtin_amdgcn_sched_barrier(0);
                 fwd_mfma(acc, S[st], odd, flip);
                 __builtin_amdgcn_sched_barrier(0);
@@ -265,24 +345,26 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
 }
 
 // Run fwd_pair with the pair's chunk count as a template argument.
-template <int NN, int SPW, int NC>
-__device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[SPW], const float* tf, const float* wl,
-                                             int e0, int de, float2* __restrict__ coeffs, const BCase& cb, int M, int R,
-                                             int s0, int nsig, long cdist, long cstride, float norm, int lane, int layout)
+template <int NN, int SPW, int NC, int PREC>
+__device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[SPW], const float* tf, const SplitTab& ts,
+                                             long toff, const float* wl, int e0, int de, float2* __restrict__ coeffs,
+                                             const BCase& cb, int M, int R, int s0, int nsig, long cdist, long cstride,
+                                             float norm, int lane, int layout)
 {
     if constexpr (NC <= (NN / 2 + 15) / 16) {
         if (nch == NC)
-            fwd_pair<NN, SPW, NC>(gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane, layout);
+            fwd_pair<NN, SPW, NC, PREC>(gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
+                                        lane, layout);
         else
-            fwd_dispatch<NN, SPW, NC + 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
-                                          lane, layout);
+            fwd_dispatch<NN, SPW, NC + 1, PREC>(nch, gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist,
+                                                cstride, norm, lane, layout);
     }
 }
 
-template <int NN, int WPB, int SPW>
+template <int NN, int WPB, int SPW, int PREC>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
                                                                 const float* __restrict__ wts32,
-                                                                const float* __restrict__ Tf,
+                                                                const float* __restrict__ Tf, SplitTab ts,
                                                                 const float2* __restrict__ G, float2* __restrict__ coeffs,
                                                                 int nsig, int nsg, long cdist, long cstride, float norm,
                                                                 int layout)
@@ -303,14 +385,16 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp)
         gp[sp] = reinterpret_cast<const float*>(G + (long)min(s0 + sp, nsig - 1) * N3 + cb.vec);
-    const float* tf = Tf + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
+    const long toff = (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
+    const float* tf = Tf + toff;
     const float* wl = wts32 + 4 * kk;
     // float offset of this lane's complex pair in beta block 0, and its step per block
     const int p = i & 1;
     const int e0 = 2 * (cb.flip ? NN - 4 - 4 * kk + 2 * (1 - p) : 4 * kk + 2 * p);
     const int de = cb.flip ? -32 : 32;
 
-    fwd_dispatch<NN, SPW, 1>(nch, gp, tf, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm, lane, layout);
+    fwd_dispatch<NN, SPW, 1, PREC>(nch, gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
+                                   lane, layout);
 }
 
 constexpr int OSTRF = 17;  // output staging row stride (floats), NN rows of 16
@@ -339,15 +423,56 @@ __device__ __forceinline__ void inv_gather(InvStage<SPW>& S, const float* const 
     }
 }
 
-template <int NKB, int SPW>
-__device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStage<SPW>& S, float4 (&a)[NKB],
-                                             const float* tnext)
+// Table operands of the inverse: FP32 tiles, or their split parts.
+template <int NKB, int PREC>
+struct InvTab {
+    float4 a[PREC ? 1 : NKB];
+    v4s as[PREC ? NKB : 1][split_parts(PREC)];
+};
+
+template <int NKB, int PREC>
+__device__ __forceinline__ void inv_tab_load(InvTab<NKB, PREC>& T, int kb, const float* tnext, const SplitTab& ts,
+                                             long tnoff)
+{
+    if constexpr (PREC == 0) {
+        T.a[kb] = *reinterpret_cast<const float4*>(tnext + kb * 256);
+    } else {
+#pragma unroll
+        for (int pp = 0; pp < split_parts(PREC); ++pp)
+            T.as[kb][pp] = *reinterpret_cast<const v4s*>(ts.p[pp] + tnoff + kb * 256);
+    }
+}
+
+template <int NKB, int SPW, int PREC>
+__device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStage<SPW>& S, InvTab<NKB, PREC>& T,
+                                             const float* tnext, const SplitTab& ts, long tnoff)
 {
     float b[SPW][4];
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp)
 #pragma unroll
         for (int u = 0; u < 4; ++u) b[sp][u] = S.v[sp][u] * S.f[u];
+    if constexpr (PREC != 0) {
+        // the four degrees of an FP32 step group are the K = 16 of one bf16 MFMA
+        v4s bs[SPW][split_parts(PREC)];
+#pragma unroll
+        for (int sp = 0; sp < SPW; ++sp) split4<split_parts(PREC)>(b[sp], bs[sp]);
+        __builtin_amdgcn_sched_barrier(0);
+        constexpr int GS = NKB < 4 ? NKB : 4;
+#pragma unroll
+        for (int g = 0; g < NKB / GS; ++g) {
+#pragma unroll
+            for (int k4 = 0; k4 < GS; ++k4)
+#pragma unroll
+                for (int sp = 0; sp < SPW; ++sp)
+                    acc[GS * g + k4][sp] = mfma_split<PREC>(T.as[GS * g + k4], bs[sp], acc[GS * g + k4][sp]);
+#pragma unroll
+            for (int k4 = 0; k4 < GS; ++k4) inv_tab_load<NKB, PREC>(T, GS * g + k4, tnext, ts, tnoff);
+            __builtin_amdgcn_sched_group_barrier(0x008, (PREC == 6 ? 6 : 3) * GS * SPW, 0);  // the group's MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x020, split_parts(PREC) * GS, 0);       // then its tile loads
+        }
+    } else {
+    float4 (&a)[NKB] = T.a;
     __builtin_amdgcn_sched_barrier(0);
     // Tiles in groups of four: a group's GS x SPW independent accumulator
     // chains hide the MFMA latency, and its registers take the next chunk's
@@ -373,6 +498,7 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
         __builtin_amdgcn_sched_group_barrier(0x008, 4 * GS * SPW, 0);  // the group's MFMAs
         __builtin_amdgcn_sched_group_barrier(0x020, GS, 0);            // then its next-chunk tile loads
     }
+    }
 }
 
 // Inverse contraction of one canonical pair for SPW signals per wave:
@@ -380,9 +506,9 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
 // A = column-major table tiles (k rows, r columns), B = the coefficients,
 // gathered per lane.  The 8 beta-vectors of each signal are staged per wave
 // in LDS and written out as contiguous pencils.
-template <int NN, int WPB, int SPW>
+template <int NN, int WPB, int SPW, int PREC>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, const long* __restrict__ roff,
-                                                                const float* __restrict__ Ti,
+                                                                const float* __restrict__ Ti, SplitTab ts,
                                                                 const float2* __restrict__ coeffs, float2* __restrict__ G,
                                                                 int nsig, int nsg, long cdist, long cstride, float norm,
                                                                 int layout)
@@ -404,7 +530,8 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp)
         cs[sp] = reinterpret_cast<const float*>(coeffs + (long)min(s0 + sp, nsig - 1) * cdist) + part;
-    const float* ti = Ti + (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
+    const long toff = (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
+    const float* ti = Ti + toff;
 
     v4f acc[NKB][SPW];
 #pragma unroll
@@ -417,15 +544,15 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
     // past R and for inactive cases, scaled by 0): a branch around them would
     // make the compiler drain all outstanding loads.
     InvStage<SPW> bc, bn;
-    float4 a[NKB];
+    InvTab<NKB, PREC> tab;
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) a[kb] = *reinterpret_cast<const float4*>(ti + kb * 256);
+    for (int kb = 0; kb < NKB; ++kb) inv_tab_load<NKB, PREC>(tab, kb, ti, ts, toff);
     inv_gather<SPW>(bc, cs, cb, M, R, cstride, norm, kk, 0);
     for (int q = 0; q < nch; ++q) {
         const int qn = q + 1 < nch ? q + 1 : q;
         inv_gather<SPW>(bn, cs, cb, M, R, cstride, norm, kk, qn);
         __builtin_amdgcn_sched_barrier(0);
-        inv_contract<NKB, SPW>(acc, bc, a, ti + qn * NKB * 256);
+        inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
         __builtin_amdgcn_sched_barrier(0);
         bc = bn;
     }
@@ -522,7 +649,14 @@ inline dim3 batch_grid(int npairs, int nsg) { return dim3((unsigned)(8 * ((npair
 // leave one wave per SIMD.
 constexpr int max_spw(int NN, bool fwd) { return NN >= 256 ? 2 : (fwd || NN <= 64 ? 4 : 2); }
 
-template <int NN, int WPB, int SPW>
+inline SplitTab split_tab(const BatchTables& bt, bool fwd)
+{
+    SplitTab ts;
+    for (int pp = 0; pp < 3; ++pp) ts.p[pp] = fwd ? bt.tf16[pp] : bt.ti16[pp];
+    return ts;
+}
+
+template <int NN, int WPB, int SPW, int PREC>
 hipError_t fwd_launch(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long cdist,
                       long cstride, float norm, hipStream_t st)
 {
@@ -530,12 +664,13 @@ hipError_t fwd_launch(const PlanTables& t, const BatchTables& bt, const float2* 
         return hipErrorInvalidValue;
     } else {
         const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
-        return pfs::launch((k_dwt_fwd_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), 0, st, t,
-                           bt.roff, bt.wts32, bt.tf, G, coeffs, nsig, nsg, cdist, cstride, norm, g_mfma_f32_layout);
+        return pfs::launch((k_dwt_fwd_batch_f32<NN, WPB, SPW, PREC>), batch_grid(t.npairs, nsg), dim3(64 * WPB), 0, st,
+                           t, bt.roff, bt.wts32, bt.tf, split_tab(bt, true), G, coeffs, nsig, nsg, cdist, cstride, norm,
+                           g_mfma_f32_layout);
     }
 }
 
-template <int NN, int WPB, int SPW>
+template <int NN, int WPB, int SPW, int PREC>
 hipError_t inv_launch(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
                       long cstride, float norm, hipStream_t st)
 {
@@ -545,31 +680,104 @@ hipError_t inv_launch(const PlanTables& t, const BatchTables& bt, const float2* 
         const size_t smem = sizeof(float) * (size_t)WPB * NN * OSTRF;
         if (smem > lds_limit()) return hipErrorInvalidConfiguration;
         const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
-        return pfs::launch((k_dwt_inv_batch_f32<NN, WPB, SPW>), batch_grid(t.npairs, nsg), dim3(64 * WPB), smem, st, t,
-                           bt.roff, bt.ti, coeffs, G, nsig, nsg, cdist, cstride, norm, g_mfma_f32_layout);
+        return pfs::launch((k_dwt_inv_batch_f32<NN, WPB, SPW, PREC>), batch_grid(t.npairs, nsg), dim3(64 * WPB), smem,
+                           st, t, bt.roff, bt.ti, split_tab(bt, false), coeffs, G, nsig, nsg, cdist, cstride, norm,
+                           g_mfma_f32_layout);
     }
 }
 
-template <int NN>
+// Split precisions run at most two signals per wave (their operands take more
+// registers), so their instances are the shapes of one and two.
+template <int NN, int PREC>
 hipError_t fwd_nn(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long cdist,
                   long cstride, float norm, BatchKernelShape sh, hipStream_t st)
 {
 #define PFS_B(W, S) \
-    if (sh.wpb == W && sh.spw == S) return fwd_launch<NN, W, S>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
-    PFS_B(1, 1) PFS_B(1, 2) PFS_B(1, 4) PFS_B(2, 1) PFS_B(2, 2) PFS_B(2, 4) PFS_B(4, 1) PFS_B(4, 2) PFS_B(4, 4)
+    if (sh.wpb == W && sh.spw == S) return fwd_launch<NN, W, S, PREC>(t, bt, G, coeffs, nsig, cdist, cstride, norm, st);
+    PFS_B(1, 1) PFS_B(1, 2) PFS_B(2, 1) PFS_B(2, 2) PFS_B(4, 1) PFS_B(4, 2)
+    if constexpr (PREC == 0) { PFS_B(1, 4) PFS_B(2, 4) PFS_B(4, 4) }
+#undef PFS_B
+    return hipErrorInvalidValue;
+}
+
+template <int NN, int PREC>
+hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
+                  long cstride, float norm, BatchKernelShape sh, hipStream_t st)
+{
+#define PFS_B(W, S) \
+    if (sh.wpb == W && sh.spw == S) return inv_launch<NN, W, S, PREC>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
+    PFS_B(1, 1) PFS_B(1, 2) PFS_B(2, 1) PFS_B(2, 2) PFS_B(4, 1) PFS_B(4, 2)
+    if constexpr (PREC == 0) { PFS_B(1, 4) PFS_B(2, 4) PFS_B(4, 4) }
 #undef PFS_B
     return hipErrorInvalidValue;
 }
 
 template <int NN>
-hipError_t inv_nn(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
-                  long cstride, float norm, BatchKernelShape sh, hipStream_t st)
+hipError_t fwd_prec(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs, int nsig, long cdist,
+                    long cstride, float norm, BatchKernelShape sh, int prec, hipStream_t st)
 {
-#define PFS_B(W, S) \
-    if (sh.wpb == W && sh.spw == S) return inv_launch<NN, W, S>(t, bt, coeffs, G, nsig, cdist, cstride, norm, st);
-    PFS_B(1, 1) PFS_B(1, 2) PFS_B(1, 4) PFS_B(2, 1) PFS_B(2, 2) PFS_B(2, 4) PFS_B(4, 1) PFS_B(4, 2) PFS_B(4, 4)
-#undef PFS_B
-    return hipErrorInvalidValue;
+    switch (prec) {
+    case 0: return fwd_nn<NN, 0>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 3: return fwd_nn<NN, 3>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 6: return fwd_nn<NN, 6>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NN>
+hipError_t inv_prec(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G, int nsig, long cdist,
+                    long cstride, float norm, BatchKernelShape sh, int prec, hipStream_t st)
+{
+    switch (prec) {
+    case 0: return inv_nn<NN, 0>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 3: return inv_nn<NN, 3>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 6: return inv_nn<NN, 6>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Bf16 terms of every FP32 table value (the tiles keep their element order).
+__global__ void k_split_tables_bf16(const float* __restrict__ T, long n, unsigned short* __restrict__ h,
+                                    unsigned short* __restrict__ m, unsigned short* __restrict__ l)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float r = T[i];
+        unsigned b = bf16_rne_bits(r);
+        h[i] = (unsigned short)(b >> 16);
+        r -= __uint_as_float(b);
+        b = bf16_rne_bits(r);
+        m[i] = (unsigned short)(b >> 16);
+        r -= __uint_as_float(b);
+        l[i] = (unsigned short)(bf16_rne_bits(r) >> 16);
+    }
+}
+
+// Self-test probe of the split path's operand packing: a 16 x 16 x 16 product
+// of small integers (exact in bf16 and FP32) as four FP32 16x16x4 MFMAs and
+// as one bf16 16x16x16 MFMA, with the K-ordered packing split4 produces.
+__global__ void k_mfma_probe_bf16(float* out)
+{
+    const int lane = threadIdx.x, i = lane & 15, kk = lane >> 4;
+    float a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int k = 4 * kk + u;
+        a[u] = (float)(((i * 7 + k * 3) % 9) - 4);   // A[i][k]
+        b[u] = (float)(((k * 5 + i * 11) % 7) - 3);  // B[k][n = i]
+    }
+    v4f f = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) f = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], f, 0, 0, 0);
+    v4s as[1], bs[1];
+    split4<1>(a, as);
+    split4<1>(b, bs);
+    v4f h = {0.f, 0.f, 0.f, 0.f};
+    h = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as[0], bs[0], h, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        out[lane * 4 + v] = f[v];
+        out[256 + lane * 4 + v] = h[v];
+    }
 }
 
 }  // namespace
@@ -607,32 +815,68 @@ bool batch_shape_valid(int N, bool fwd, BatchKernelShape sh)
            sh.spw <= smax;
 }
 
-hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
-                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh, hipStream_t st)
+// The default shape for a batch, with at most two signals per wave in the
+// split precisions.
+static BatchKernelShape prec_shape(int N, bool fwd, int nsig, BatchKernelShape sh, int prec)
 {
-    if (sh.wpb == 0) sh = batch_default_shape(t.N, true, nsig);
+    if (sh.wpb == 0) sh = batch_default_shape(N, fwd, nsig);
+    if (prec != 0 && sh.spw > 2) sh.spw = 2;
+    return sh;
+}
+
+hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
+                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh, hipStream_t st,
+                                    int prec)
+{
+    if (prec != 0 && (!g_mfma_bf16_ok || !bt.tf16[0])) return hipErrorInvalidValue;
+    sh = prec_shape(t.N, true, nsig, sh, prec);
     switch (t.N) {
-    case 16: return fwd_nn<16>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
-    case 32: return fwd_nn<32>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
-    case 64: return fwd_nn<64>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
-    case 128: return fwd_nn<128>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
-    case 256: return fwd_nn<256>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, st);
+    case 16: return fwd_prec<16>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, prec, st);
+    case 32: return fwd_prec<32>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, prec, st);
+    case 64: return fwd_prec<64>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, prec, st);
+    case 128: return fwd_prec<128>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, prec, st);
+    case 256: return fwd_prec<256>(t, bt, G, coeffs, nsig, cdist, cstride, norm, sh, prec, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
-                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh, hipStream_t st)
+                                    int nsig, long cdist, long cstride, float norm, BatchKernelShape sh, hipStream_t st,
+                                    int prec)
 {
-    if (sh.wpb == 0) sh = batch_default_shape(t.N, false, nsig);
+    if (prec != 0 && (!g_mfma_bf16_ok || !bt.ti16[0])) return hipErrorInvalidValue;
+    sh = prec_shape(t.N, false, nsig, sh, prec);
     switch (t.N) {
-    case 16: return inv_nn<16>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
-    case 32: return inv_nn<32>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
-    case 64: return inv_nn<64>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
-    case 128: return inv_nn<128>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
-    case 256: return inv_nn<256>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, st);
+    case 16: return inv_prec<16>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, prec, st);
+    case 32: return inv_prec<32>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, prec, st);
+    case 64: return inv_prec<64>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, prec, st);
+    case 128: return inv_prec<128>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, prec, st);
+    case 256: return inv_prec<256>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, prec, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_split_tables_bf16(const float* T, long n, unsigned short* h, unsigned short* m, unsigned short* l,
+                                    hipStream_t st)
+{
+    const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 16384);
+    return pfs::launch(k_split_tables_bf16, dim3(blocks), dim3(256), 0, st, T, n, h, m, l);
+}
+
+int bf16_selftest(hipStream_t st)
+{
+    float* d = nullptr;
+    if (hipMalloc(&d, 512 * sizeof(float)) != hipSuccess) return -1;
+    hipError_t e = pfs::launch(k_mfma_probe_bf16, dim3(1), dim3(64), 0, st, d);
+    float h[512];
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return -1;
+    bool same = true;
+    for (int i = 0; i < 256; ++i) same = same && h[i] == h[256 + i];
+    g_mfma_bf16_ok = same;
+    return same ? 0 : 1;
 }
 
 hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float2* interleaved, long count, int n,

@@ -226,6 +226,9 @@ struct Plan {
     // FP32 Wigner tables of the batched path (built on first use)
     void* d32 = nullptr;  // row offsets, FP32 weights and both tile tables
     size_t d32_bytes = 0;
+    // bf16 terms of both tile tables for the split precisions (built on first use)
+    void* d16 = nullptr;
+    size_t d16_bytes = 0;
     BatchTables bt{};
     // batched contraction shapes picked by the autotuner per direction and
     // batch-size bucket (ceil(log2 n), capped at 7); {0, 0} = default rule
@@ -270,6 +273,9 @@ struct pfs_context {
     long fft_chunk_slices = 0;
     long batch_chunk_signals = 64;
     long batch_interleave = 1;  // contraction kernels on signal-interleaved coefficients
+    // batched contraction precision: 0 FP32 MFMA (default), 3 / 6 split-bf16
+    // products on bf16 MFMA with FP32 accumulation
+    long batch_precision = 0;
     long dwt_variant = 0;       // work shape of the FP64 DWT kernels (2B <= 256)
     // inverse DWT kernel: 0 MFMA; vector ALU with 2 chains x 2 waves over the
     // nodes, 4 chains x 2 waves over the right-hand sides, or 4 x 1 per pair
@@ -842,6 +848,30 @@ int ensure_table32(pfs_context* c, Plan& p, hipStream_t st)
     return PFS_OK;
 }
 
+// The bf16 terms (hi, mid, lo) of both FP32 tile tables, for the split-bf16
+// contraction: 3 x 2 bytes per table value and table.
+int ensure_table_bf16(pfs_context* c, Plan& p, hipStream_t st)
+{
+    if (p.d16) return PFS_OK;
+    if (!g_mfma_bf16_ok)
+        return fail(PFS_ERR_UNSUPPORTED, "split-bf16 contraction: the bf16 MFMA self-test did not pass on this device");
+    const long n = batch_table_rows(p.B) * p.t.N;
+    const size_t part = ((size_t)n * sizeof(unsigned short) + 255) & ~(size_t)255;
+    int rc = ctx_alloc(c, &p.d16, 6 * part);
+    if (rc) return rc;
+    p.d16_bytes = 6 * part;
+    unsigned short* q[6];
+    for (int i = 0; i < 6; ++i) q[i] = (unsigned short*)((char*)p.d16 + i * part);
+    HIP_TRY(launch_split_tables_bf16(p.bt.tf, n, q[0], q[1], q[2], st));
+    HIP_TRY(launch_split_tables_bf16(p.bt.ti, n, q[3], q[4], q[5], st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < 3; ++i) {
+        p.bt.tf16[i] = q[i];
+        p.bt.ti16[i] = q[3 + i];
+    }
+    return PFS_OK;
+}
+
 int ensure_bwork(pfs_context* c, size_t bytes)
 {
     if (c->bwork_bytes >= bytes) return PFS_OK;
@@ -869,6 +899,10 @@ int batch_prepare(pfs_context* c, int B, int nsig, Plan** pp, int* sb, hipStream
         return fail(PFS_ERR_UNSUPPORTED, "batched mixed-precision path supports B = 8, 16, 32, 64, 128 (got %d)", B);
     rc = ensure_table32(c, **pp, st);
     if (rc) return rc;
+    if (c->batch_precision) {
+        rc = ensure_table_bf16(c, **pp, st);
+        if (rc) return rc;
+    }
     *sb = (int)std::min<long>(c->batch_chunk_signals, nsig);
     const size_t N = 2 * (size_t)B;
     return ensure_bwork(c, (size_t)*sb * N * N * N * sizeof(float2));
@@ -999,10 +1033,10 @@ int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSe
     if (c->batch_interleave || coeffs.ptrs) {
         // W is free once the column pass has run: the coefficients go there interleaved
         const int ld = interleave_ld(n, cap, count, N3);
-        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, shape_for(c, *p, true, n), st); }));
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, W, n, 1, ld, norm, shape_for(c, *p, true, n), st, (int)c->batch_precision); }));
         HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs.base, coeffs.ptrs, W, count, n, ld, false, st); }));
     } else {
-        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs.base, n, coeffs.stride, 1, norm, shape_for(c, *p, true, n), st); }));
+        HIP_TRY(timed(c, "batch_dwt_fwd", st, [&] { return launch_dwt_fwd_batch_f32(p->t, p->bt, G, coeffs.base, n, coeffs.stride, 1, norm, shape_for(c, *p, true, n), st, (int)c->batch_precision); }));
     }
     return PFS_OK;
 }
@@ -1020,9 +1054,9 @@ int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, Sig
     if (c->batch_interleave || coeffs.ptrs) {
         const int ld = interleave_ld(n, cap, count, N3);
         HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(coeffs.base, coeffs.ptrs, W, count, n, ld, true, st); }));
-        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, shape_for(c, *p, false, n), st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, W, G, n, 1, ld, norm, shape_for(c, *p, false, n), st, (int)c->batch_precision); }));
     } else {
-        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, shape_for(c, *p, false, n), st); }));
+        HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, shape_for(c, *p, false, n), st, (int)c->batch_precision); }));
     }
     const int S = batch_fft_signals(c, n), K = batch_fft_slices(c, N);
     const long n2 = (long)N * N;
@@ -1134,6 +1168,12 @@ int pfs_ctx_create(int device, size_t hbm_budget_bytes, pfs_context** out)
     }
     int rc = mfma_selftest(c->stream);
     if (!rc) rc = mfma_selftest_f32(c->stream);
+    // the split-bf16 contraction is available only if its operand packing
+    // reproduces the FP32 MFMA's products here (checked once per process)
+    if (!rc) {
+        static std::once_flag bf16_once;
+        std::call_once(bf16_once, [&] { (void)pfs::bf16_selftest(c->stream); });
+    }
     // the device address of this device's allocation-free scratch array (once
     // per device; a failure is reported by the entry point that needs it)
     if (!rc) pfs::noalloc_setup(device);
@@ -1156,6 +1196,7 @@ int pfs_ctx_destroy(pfs_context* c)
         for (auto& kv : c->plans) {
             ctx_free(c, kv.second.mem, kv.second.bytes);
             ctx_free(c, kv.second.d32, kv.second.d32_bytes);
+            ctx_free(c, kv.second.d16, kv.second.d16_bytes);
         }
         ctx_free(c, c->bwork[0], c->bwork_bytes);
         ctx_free(c, c->bwork[1], c->bwork_bytes);
@@ -1430,6 +1471,13 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->batch_interleave = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "batch_precision")) {
+        if (value != 0 && value != 3 && value != 6) return fail(PFS_ERR_ARG, "batch_precision must be 0, 3 or 6");
+        if (value && !g_mfma_bf16_ok)
+            return fail(PFS_ERR_UNSUPPORTED, "batch_precision %ld: the bf16 MFMA self-test did not pass", value);
+        c->batch_precision = value;
+        return PFS_OK;
+    }
     if (!strcmp(key, "dwt_inv_kernel")) {
         if (value < 0 || value > 7 || value == 4)
             return fail(PFS_ERR_ARG, "dwt_inv_kernel must be 0 (MFMA), 1..3 or 5..7 (vector ALU)");
@@ -1490,6 +1538,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_interleave")) {
         *value = c->batch_interleave;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_precision")) {
+        *value = c->batch_precision;
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_inv_kernel")) {
@@ -1649,7 +1701,9 @@ int batch_warm(pfs_context* c, int B)
     if (rc) return rc;
     if (!batch_supported(p->t.N))
         return fail(PFS_ERR_UNSUPPORTED, "batched mixed-precision path supports B = 8, 16, 32, 64, 128 (got %d)", B);
-    return ensure_table32(c, *p, c->stream);
+    rc = ensure_table32(c, *p, c->stream);
+    if (!rc && c->batch_precision) rc = ensure_table_bf16(c, *p, c->stream);
+    return rc;
 }
 
 size_t batch_work_bytes(int B, int cap)

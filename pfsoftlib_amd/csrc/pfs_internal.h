@@ -192,7 +192,18 @@ struct BatchTables {
     const float* wts32;  // quadrature weights rounded to FP32, N values
     const float* tf;     // Wigner-d tiles, row-major within a tile
     const float* ti;     // the same tiles, column-major within a tile
+    // bf16 terms (hi, mid, lo) of tf and ti for the split precisions, same
+    // element order; null until a split precision is first used
+    const unsigned short* tf16[3];
+    const unsigned short* ti16[3];
 };
+// Whether the context self-test found the bf16 MFMA operand packing of the
+// split precisions to give the FP32 MFMA's products (set by bf16_selftest).
+extern bool g_mfma_bf16_ok;
+int bf16_selftest(hipStream_t st);
+// Bf16 terms h, m, l of n FP32 table values.
+hipError_t launch_split_tables_bf16(const float* T, long n, unsigned short* h, unsigned short* m, unsigned short* l,
+                                    hipStream_t st);
 // Number of table rows (16-padded per pair) for bandwidth B.
 long batch_table_rows(int B);
 hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st);
@@ -206,12 +217,15 @@ BatchKernelShape batch_default_shape(int N, bool fwd, int nsig);
 bool batch_shape_valid(int N, bool fwd, BatchKernelShape sh);
 // Coefficient (s, idx) of the batch sits at s * cdist + idx * cstride (complex
 // elements): (count, 1) is the per-signal layout, (1, ld) the interleaved one.
+// prec: 0 FP32 MFMA; 3 / 6 split-bf16 products (three or six bf16 MFMAs per
+// FP32 step group, FP32 accumulation; needs the bf16 tables and a passing
+// bf16 self-test).
 hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* G, float2* coeffs,
                                     int nsig, long cdist, long cstride, float norm, BatchKernelShape sh,
-                                    hipStream_t st);
+                                    hipStream_t st, int prec = 0);
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
                                     int nsig, long cdist, long cstride, float norm, BatchKernelShape sh,
-                                    hipStream_t st);
+                                    hipStream_t st, int prec = 0);
 // Between the per-signal layout (per_signal + s * count, or sptr[s] when sptr
 // is set) and the interleaved layout [idx][ld], in the direction given.
 hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float2* interleaved, long count, int n,

@@ -160,3 +160,58 @@ def test_chunked_batch_fft_is_bitwise_invariant(ctx, torch, B, nsig):
     torch.cuda.synchronize()
     for g, c in outs[1:]:
         assert torch.equal(g, outs[0][0]) and torch.equal(c, outs[0][1])
+
+
+@pytest.mark.parametrize("prec,bound", [(6, 1e-6), (3, 2e-5)])
+@pytest.mark.parametrize("B,nsig", [(16, 5), (64, 6), (128, 2)])
+def test_split_bf16_contraction_against_fp64_oracle(ctx, torch, B, nsig, prec, bound):
+    """The split-bf16 contractions (batch_precision 3 and 6: three or six bf16
+    MFMAs per FP32 step group, FP32 accumulation) against the FP64 oracle per
+    signal and direction.  bf16 x6 keeps the 1e-6 bar of the batched path;
+    bf16 x3 drops terms of order 2^-16 and is held to 2e-5."""
+    N = 2 * B
+    s = torch.cuda.current_stream()
+    cs = [O.seeded_coeffs(B, 900 + i) for i in range(nsig)]
+    cdev = torch.from_numpy(np.concatenate(cs).astype(np.complex64)).cuda()
+    g = torch.empty(nsig * N ** 3, dtype=torch.complex64, device="cuda")
+    back = torch.empty_like(cdev)
+    ctx.tune("batch_precision", prec)
+    try:
+        ctx.idsoft_batch_f32(B, nsig, cdev, g, stream=s)
+        ctx.dsoft_batch_f32(B, nsig, g, back, stream=s)
+        torch.cuda.synchronize()
+    finally:
+        ctx.tune("batch_precision", 0)
+    gh = g.cpu().numpy().reshape(nsig, -1)
+    rh = back.cpu().numpy().reshape(nsig, -1)
+    worst = 0.0
+    for i in range(nsig):
+        e_inv = rel(gh[i].astype(np.complex128), O.idsoft(B, cs[i]))
+        e_fwd = rel(rh[i].astype(np.complex128), O.dsoft(B, gh[i].astype(np.complex128)))
+        worst = max(worst, e_inv, e_fwd)
+        assert e_inv < bound and e_fwd < bound, (prec, i, e_inv, e_fwd)
+    print("bf16 x%d B=%d: worst rel L2 vs FP64 oracle %.3e" % (prec, B, worst))
+
+
+def test_split_bf16_signal_is_same_alone_and_in_batch(ctx, torch):
+    B, N, nsig = 32, 64, 7
+    n = O.coeff_count(B)
+    cs = np.concatenate([O.seeded_coeffs(B, 70 + i) for i in range(nsig)]).astype(np.complex64)
+    cdev = torch.from_numpy(cs).cuda()
+    g_all = torch.empty(nsig * N ** 3, dtype=torch.complex64, device="cuda")
+    g_one = torch.empty(N ** 3, dtype=torch.complex64, device="cuda")
+    ctx.tune("batch_precision", 6)
+    try:
+        ctx.idsoft_batch_f32(B, nsig, cdev, g_all)
+        ctx.idsoft_batch_f32(B, 1, cdev[4 * n:5 * n], g_one)
+        torch.cuda.synchronize()
+    finally:
+        ctx.tune("batch_precision", 0)
+    assert torch.equal(g_all[4 * N ** 3:5 * N ** 3], g_one)
+
+
+def test_batch_precision_key_rejects_other_values(ctx, torch):
+    import pfsoftlib_amd as P
+    with pytest.raises(P.PfsError):
+        ctx.tune("batch_precision", 4)
+    assert ctx.tune_get("batch_precision") == 0
