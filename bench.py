@@ -435,6 +435,121 @@ def serving_leg(torch, P, ctx, seconds, seed, budget_bytes, slots, max_batch, ut
     }
 
 
+def multi_batch_leg(torch, P, ctx, seconds, seed, rate, max_batch=256, burst=2000):
+    """The serving mix through the mixed-bandwidth batch entry points
+    (pfs_{d,id}soft_batch_multi_f32_dev) on one stream, continuous batching on
+    the host: whenever the previous batch is done, every request that has
+    arrived (up to max_batch) goes into the next batch, forward and inverse
+    requests as one call each.  Reports capacity (a saturating burst), p50/p99
+    latency (arrival to batch completion) at the offered rate `rate` -- the
+    slot server's offered rate, so the two designs are compared at the same
+    load -- the error of every bandwidth and kind against the FP64 path, and
+    the fixed-composition batch time with and without graph replay."""
+    import numpy as np
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(seed + 7)
+    nin, ring = 4, {16: 512, 32: 512, 64: 160, 128: 64}
+    inputs, outs = {}, {}
+    for B in SERVE_BANDWIDTHS:
+        N, n = 2 * B, P.coeff_count(B)
+        c = P.seeded_coeffs_torch(B, [seed + 100 * B + i for i in range(nin)], "cuda", dtype=torch.complex64)
+        g = torch.empty((nin, N ** 3), dtype=torch.complex64, device="cuda")
+        ctx.idsoft_batch_f32(B, nin, c, g, stream=stream)
+        inputs[(B, P.PFS_REQ_INVERSE)], inputs[(B, P.PFS_REQ_FORWARD)] = c, g
+        outs[(B, P.PFS_REQ_INVERSE)] = torch.empty((ring[B], N ** 3), dtype=torch.complex64, device="cuda")
+        outs[(B, P.PFS_REQ_FORWARD)] = torch.empty((ring[B], n), dtype=torch.complex64, device="cuda")
+    torch.cuda.synchronize()
+    pos = {k: 0 for k in outs}
+    last = {}
+
+    def draw():
+        return int(rng.choice(SERVE_BANDWIDTHS, p=SERVE_PROBS)), int(rng.integers(0, 2))
+
+    def launch(reqs):
+        done_ev = torch.cuda.Event()
+        for kind, call in ((P.PFS_REQ_FORWARD, ctx.dsoft_batch_multi), (P.PFS_REQ_INVERSE, ctx.idsoft_batch_multi)):
+            sel = [r for r in reqs if r[1] == kind]
+            if not sel:
+                continue
+            bws, ins, os_ = [], [], []
+            for (B, k, j) in sel:
+                r = pos[(B, k)]
+                pos[(B, k)] = (r + 1) % ring[B]
+                bws.append(B)
+                ins.append(inputs[(B, k)][j])
+                os_.append(outs[(B, k)][r])
+                last[(B, k)] = (r, j)
+            call(bws, ins, os_, stream=stream)
+        done_ev.record(stream)
+        return done_ev
+
+    ctx.tune("batch_graphs", 0)  # compositions change batch to batch
+    # capacity: a saturating burst, batches of max_batch
+    reqs = [draw() + (i % nin,) for i in range(burst)]
+    t0 = time.perf_counter()
+    for b0 in range(0, burst, max_batch):
+        launch(reqs[b0:b0 + max_batch])
+    torch.cuda.synchronize()
+    cap = burst / (time.perf_counter() - t0)
+    # the Poisson stream at the offered rate
+    lam = rate or 0.7 * cap
+    arr = np.cumsum(rng.exponential(1.0 / lam, size=int(lam * seconds * 1.2) + 16))
+    arr = arr[arr < seconds]
+    kinds = [draw() for _ in range(len(arr))]
+    lat, nb, i, inflight = [], 0, 0, None
+    t0 = time.perf_counter()
+    while i < len(arr) or inflight is not None:
+        now = time.perf_counter() - t0
+        if inflight is not None and inflight[0].query():
+            lat.extend(now - a for a in inflight[1])
+            inflight = None
+        if inflight is None and i < len(arr) and arr[i] <= now:
+            j = i
+            while j < len(arr) and arr[j] <= now and j - i < max_batch:
+                j += 1
+            ev = launch([kinds[k] + (k % nin,) for k in range(i, j)])
+            inflight = (ev, list(arr[i:j]))
+            nb += 1
+            i = j
+    t1 = time.perf_counter()
+    lat_ms = sorted(1e3 * x for x in lat)
+    # error of each bandwidth and kind's last result against the FP64 path
+    errs = {}
+    name = {P.PFS_REQ_FORWARD: "dsoft", P.PFS_REQ_INVERSE: "idsoft"}
+    for (B, kind), (r, j) in sorted(last.items()):
+        out = outs[(B, kind)][r].to(torch.complex128)
+        if kind == P.PFS_REQ_INVERSE:
+            ref = torch.zeros((2 * B) ** 3, dtype=torch.complex128, device="cuda")
+            ctx.idsoft_dev(B, inputs[(B, kind)][j].to(torch.complex128), ref, stream=stream)
+        else:
+            ref = torch.empty(P.coeff_count(B), dtype=torch.complex128, device="cuda")
+            ctx.dsoft_dev(B, inputs[(B, kind)][j].to(torch.complex128), ref, stream=stream)
+        errs["%s_B%d" % (name[kind], B)] = (torch.linalg.vector_norm(out - ref) / torch.linalg.vector_norm(ref)).item()
+    # one fixed composition (64 requests of the mix), graph replay against direct launches
+    fixed = [draw() + (k % nin,) for k in range(64)]
+    fixed_ms = {}
+    for graphs in (0, 1):
+        ctx.tune("batch_graphs", graphs)
+        launch(fixed)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(10):
+            launch(fixed)
+        torch.cuda.synchronize()
+        fixed_ms["graphs" if graphs else "direct"] = (time.perf_counter() - t) / 10 * 1e3
+    ctx.tune("batch_graphs", 1)
+    return {
+        "design": "mixed-bandwidth batch entry points, one stream, host-side continuous batching",
+        "max_batch": max_batch, "capacity_per_s": cap, "offered_rate_per_s": lam,
+        "requests": int(len(arr)), "completed": len(lat), "seconds": t1 - t0, "batches": nb,
+        "transforms_per_s": len(lat) / (t1 - t0),
+        "latency_ms": {"p50": lat_ms[len(lat_ms) // 2], "p99": lat_ms[min(len(lat_ms) - 1, int(0.99 * len(lat_ms)))],
+                       "max": lat_ms[-1]},
+        "rel_l2_vs_fp64": errs,
+        "fixed_64_request_batch_ms": fixed_ms,
+    }
+
+
 def dwt_accuracy(ctx, np, max_bw, runs, seed):
     """DWT round-trip accuracy on the device's Wigner-d tables, as the reference's
     benchmark/benchmark_dwt_accuracy.cpp measures it: for BW = 2, 4, ..., max_bw and
@@ -685,6 +800,9 @@ def main():
     if args.serve_seconds > 0:
         out["serving"] = serving_leg(torch, P, ctx, args.serve_seconds, args.seed, int(args.serve_budget_gb * 2 ** 30),
                                      args.serve_slots, args.serve_max_batch, rate=args.serve_rate or None)
+        torch.cuda.empty_cache()
+        out["serving_multi_batch"] = multi_batch_leg(torch, P, ctx, args.serve_seconds, args.seed,
+                                                     out["serving"]["offered_rate_per_s"])
     out["dwt_accuracy"] = dwt_accuracy(ctx, np, 256, 10, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)

@@ -122,6 +122,26 @@ int pfs_dsoft_batch_f32_dev(pfs_context* ctx, int bandwidth, int nsig, const voi
 int pfs_idsoft_batch_f32_dev(pfs_context* ctx, int bandwidth, int nsig, const void* coeffs_dev, void* grids_dev,
                              void* stream);
 
+/* Batched mixed-precision transforms of nsig signals of mixed bandwidth in
+ * one call (no counterpart in the reference, which transforms one grid of one
+ * bandwidth per call).  bandwidths[i] (host array) is signal i's bandwidth
+ * (8, 16, 32, 64 or 128); its input and output are at the device addresses
+ * in[i] and out[i] (host arrays of device pointers): complex64 grids
+ * (reference layout) and packed complex64 coefficients as in
+ * pfs_dsoft_batch_f32_dev.  The slice FFTs and coefficient transposes run per
+ * bandwidth group; the Wigner contraction of all groups is one persistent
+ * launch over a work queue of (bandwidth, pair, signal group) items, heaviest
+ * first.  The launch sequence of each batch composition (direction and the
+ * signal count of each bandwidth) is captured once into a HIP graph and
+ * replayed after that (context tune key "batch_graphs", 1 default, 0 =
+ * launch directly).  Results equal those of the single-bandwidth entry
+ * points bit for bit.  Enqueued on stream; the address tables are uploaded
+ * on that stream before the batch. */
+int pfs_dsoft_batch_multi_f32_dev(pfs_context* ctx, int nsig, const int* bandwidths, const void* const* grids_dev,
+                                  void* const* coeffs_dev, void* stream);
+int pfs_idsoft_batch_multi_f32_dev(pfs_context* ctx, int nsig, const int* bandwidths, const void* const* coeffs_dev,
+                                   void* const* grids_dev, void* stream);
+
 /* Autotuner of the batched path (no counterpart in the reference): times every
  * work shape of the two Wigner contraction kernels (waves per workgroup x
  * signals per wave) for batches of 1, 2, 4, ... max_signals signals at this
@@ -211,6 +231,8 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *                       Cache at the price of more, smaller launches.
  *   "batch_chunk_signals" signals per sub-batch of the batched path (work
  *                       buffers hold this many complex64 grids; default 64).
+ *   "batch_graphs"      mixed-bandwidth batches replay a captured HIP graph
+ *                       per composition (1, default) or launch directly (0).
  *   "batch_precision"   products of the batched Wigner contraction: 0 FP32
  *                       MFMA (default); 3 split-bf16 x3 (h h' + h m' + m h'
  *                       on bf16 MFMA, FP32 accumulation); 6 split-bf16 x6

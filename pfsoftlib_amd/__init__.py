@@ -112,6 +112,10 @@ def lib():
             "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_memcpy_d2h": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_synchronize": (ip, [vp, vp]),
+            "pfs_dsoft_batch_multi_f32_dev": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(vp),
+                                                  ctypes.POINTER(vp), vp]),
+            "pfs_idsoft_batch_multi_f32_dev": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(vp),
+                                                   ctypes.POINTER(vp), vp]),
             "pfs_batch_autotune": (ip, [vp, ip, ip]),
             "pfs_batch_shape": (ip, [vp, ip, ip, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]),
             "pfs_server_create": (ip, [vp, ctypes.POINTER(ip), ip, ip, ip, ctypes.c_size_t, ctypes.POINTER(vp)]),
@@ -295,6 +299,28 @@ class Context:
         n = max(int(nsig), 0)
         _check(lib().pfs_idsoft_batch_f32_dev(self._h, B, nsig, self._coef(coeffs_dev, "coeffs", B, "c64", n),
                                               self._grid(grids_dev, "grids", B, "c64", n), _stream_handle(stream)))
+
+    def _multi(self, fwd, bandwidths, ins, outs, stream):
+        n = len(bandwidths)
+        if len(ins) != n or len(outs) != n:
+            raise PfsArgumentError("one input and one output per signal")
+        bw = (ctypes.c_int * n)(*[int(b) for b in bandwidths])
+        pin, pout = (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)()
+        for i, (b, x, y) in enumerate(zip(bandwidths, ins, outs)):
+            ng, nc = (2 * b) ** 3, coeff_count(b)
+            pin[i] = _devbuf(x, "in[%d]" % i, "c64", ng if fwd else nc, self.device)
+            pout[i] = _devbuf(y, "out[%d]" % i, "c64", nc if fwd else ng, self.device)
+        f = lib().pfs_dsoft_batch_multi_f32_dev if fwd else lib().pfs_idsoft_batch_multi_f32_dev
+        _check(f(self._h, n, bw, pin, pout, _stream_handle(stream)))
+
+    def dsoft_batch_multi(self, bandwidths, grids, coeffs, stream=None):
+        """Forward transforms of complex64 signals of mixed bandwidth: signal i
+        (bandwidth bandwidths[i]) from grids[i] into coeffs[i], one call."""
+        self._multi(True, bandwidths, grids, coeffs, stream)
+
+    def idsoft_batch_multi(self, bandwidths, coeffs, grids, stream=None):
+        """Inverse transforms of mixed bandwidth: coeffs[i] -> grids[i]."""
+        self._multi(False, bandwidths, coeffs, grids, stream)
 
     def autotune_batch(self, B, max_signals):
         """Times the batched contraction shapes for 1, 2, 4, ... max_signals signals."""

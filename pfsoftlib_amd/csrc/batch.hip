@@ -361,16 +361,15 @@ __device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[S
     }
 }
 
+// One work item of the forward contraction: pair bw.pi for signal group
+// bw.sg (WPB waves x SPW signals).
 template <int NN, int WPB, int SPW, int PREC>
-__global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
-                                                                const float* __restrict__ wts32,
-                                                                const float* __restrict__ Tf, SplitTab ts,
-                                                                const float2* __restrict__ G, float2* __restrict__ coeffs,
-                                                                int nsig, int nsg, long cdist, long cstride, float norm,
-                                                                int layout)
+__device__ __forceinline__ void fwd_item(const PlanTables& t, const long* __restrict__ roff,
+                                         const float* __restrict__ wts32, const float* __restrict__ Tf,
+                                         const SplitTab& ts, const float2* __restrict__ G, float2* __restrict__ coeffs,
+                                         int nsig, long cdist, long cstride, float norm, int layout, BlockWork bw)
 {
     constexpr int NKB = NN / 16;
-    const BlockWork bw = block_work(nsg);
     if (bw.pi >= t.npairs) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s0 = (bw.sg * WPB + w) * SPW;
@@ -395,6 +394,18 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, co
 
     fwd_dispatch<NN, SPW, 1, PREC>(nch, gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
                                    lane, layout);
+}
+
+template <int NN, int WPB, int SPW, int PREC>
+__global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
+                                                                const float* __restrict__ wts32,
+                                                                const float* __restrict__ Tf, SplitTab ts,
+                                                                const float2* __restrict__ G, float2* __restrict__ coeffs,
+                                                                int nsig, int nsg, long cdist, long cstride, float norm,
+                                                                int layout)
+{
+    fwd_item<NN, WPB, SPW, PREC>(t, roff, wts32, Tf, ts, G, coeffs, nsig, cdist, cstride, norm, layout,
+                                 block_work(nsg));
 }
 
 constexpr int OSTRF = 17;  // output staging row stride (floats), NN rows of 16
@@ -506,16 +517,14 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
 // A = column-major table tiles (k rows, r columns), B = the coefficients,
 // gathered per lane.  The 8 beta-vectors of each signal are staged per wave
 // in LDS and written out as contiguous pencils.
+// One work item of the inverse contraction; smf: WPB x NN x OSTRF floats of LDS.
 template <int NN, int WPB, int SPW, int PREC>
-__global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, const long* __restrict__ roff,
-                                                                const float* __restrict__ Ti, SplitTab ts,
-                                                                const float2* __restrict__ coeffs, float2* __restrict__ G,
-                                                                int nsig, int nsg, long cdist, long cstride, float norm,
-                                                                int layout)
+__device__ __forceinline__ void inv_item(const PlanTables& t, const long* __restrict__ roff,
+                                         const float* __restrict__ Ti, const SplitTab& ts,
+                                         const float2* __restrict__ coeffs, float2* __restrict__ G, int nsig,
+                                         long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf)
 {
     constexpr int NKB = NN / 16;
-    extern __shared__ __align__(16) float smf[];
-    const BlockWork bw = block_work(nsg);
     if (bw.pi >= t.npairs) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s0 = (bw.sg * WPB + w) * SPW;
@@ -583,6 +592,57 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template <int NN, int WPB, int SPW, int PREC>
+__global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, const long* __restrict__ roff,
+                                                                const float* __restrict__ Ti, SplitTab ts,
+                                                                const float2* __restrict__ coeffs, float2* __restrict__ G,
+                                                                int nsig, int nsg, long cdist, long cstride, float norm,
+                                                                int layout)
+{
+    extern __shared__ __align__(16) float smf[];
+    inv_item<NN, WPB, SPW, PREC>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, block_work(nsg), smf);
+}
+
+// ---- contraction over a batch of mixed bandwidths -----------------------------
+// One persistent launch per direction for signals of several bandwidths: every
+// workgroup walks a host-built queue of work items (bandwidth group, pair,
+// signal group), items b, b + gridDim.x, ...  The host orders the queue
+// heaviest first across all groups and puts the signal groups of one pair in
+// positions of one residue mod 8, so (with a grid that is a multiple of 8 and
+// round-robin dispatch) they share one XCD's L2 copy of the pair's tiles.
+// Each item runs the single-bandwidth contraction body of its group's 2B.
+template <bool FWD, int NN>
+__device__ __forceinline__ void multi_item(const BatchMultiGroup& g, BlockWork bw, float* smf, int layout)
+{
+    if constexpr (FWD)
+        fwd_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.wts32, g.tab, SplitTab{}, (const float2*)g.in, g.out,
+                                              g.nsig, g.cdist, g.cstride, g.norm, layout, bw);
+    else
+        inv_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.tab, SplitTab{}, (const float2*)g.in, g.out, g.nsig,
+                                              g.cdist, g.cstride, g.norm, layout, bw, smf);
+}
+
+template <bool FWD, int NNMAX>
+__global__ __launch_bounds__(64 * MULTI_WPB) void k_dwt_batch_multi(const BatchMultiGroup* __restrict__ groups,
+                                                                    const int4* __restrict__ items, int nitems,
+                                                                    int layout)
+{
+    extern __shared__ __align__(16) float smf[];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int4 w = items[it];
+        if (w.x < 0) continue;  // padding
+        const BatchMultiGroup& g = groups[w.x];
+        const BlockWork bw{w.y, w.z};
+        switch (g.t.N) {
+        case 16: multi_item<FWD, 16>(g, bw, smf, layout); break;
+        case 32: if constexpr (NNMAX >= 32) multi_item<FWD, 32>(g, bw, smf, layout); break;
+        case 64: if constexpr (NNMAX >= 64) multi_item<FWD, 64>(g, bw, smf, layout); break;
+        case 128: if constexpr (NNMAX >= 128) multi_item<FWD, 128>(g, bw, smf, layout); break;
+        default: if constexpr (NNMAX >= 256) multi_item<FWD, 256>(g, bw, smf, layout); break;
+        }
     }
 }
 
@@ -854,6 +914,24 @@ hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, 
     case 256: return inv_prec<256>(t, bt, coeffs, G, nsig, cdist, cstride, norm, sh, prec, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_dwt_batch_multi(bool fwd, const BatchMultiGroup* groups, const int4* items, int nitems, int nnmax,
+                                  int grid, hipStream_t st)
+{
+    if (nitems < 1 || grid < 1) return hipErrorInvalidValue;
+    const size_t smem = fwd ? 0 : sizeof(float) * (size_t)MULTI_WPB * nnmax * OSTRF;
+    if (smem > lds_limit()) return hipErrorInvalidConfiguration;
+    const dim3 g((unsigned)grid), b(64 * MULTI_WPB);
+#define PFS_M(NNM)                                                                                           \
+    if (nnmax <= NNM)                                                                                        \
+        return fwd ? pfs::launch((k_dwt_batch_multi<true, NNM>), g, b, smem, st, groups, items, nitems,     \
+                                 g_mfma_f32_layout)                                                         \
+                   : pfs::launch((k_dwt_batch_multi<false, NNM>), g, b, smem, st, groups, items, nitems,    \
+                                 g_mfma_f32_layout);
+    PFS_M(32) PFS_M(64) PFS_M(128) PFS_M(256)
+#undef PFS_M
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_split_tables_bf16(const float* T, long n, unsigned short* h, unsigned short* m, unsigned short* l,

@@ -297,6 +297,16 @@ struct pfs_context {
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
+    // mixed-bandwidth batches: group descriptors, work items and per-signal
+    // address tables in device memory, filled from a pinned host copy; the
+    // event of the last upload; the captured pipelines by batch composition
+    void* mb_dev = nullptr;
+    void* mb_host = nullptr;
+    size_t mb_bytes = 0;
+    hipEvent_t mb_ev = nullptr;
+    hipStream_t mb_cap = nullptr;
+    std::map<std::string, hipGraphExec_t> mb_graphs;
+    long batch_graphs = 1;  // mixed-bandwidth batches replay captured HIP graphs
 };
 
 LaunchTiming& pfs::pending_timing()
@@ -872,10 +882,19 @@ int ensure_table_bf16(pfs_context* c, Plan& p, hipStream_t st)
     return PFS_OK;
 }
 
+// Captured mixed-bandwidth pipelines name the work buffers and device tables
+// directly: they are dropped whenever either moves.
+void drop_multi_graphs(pfs_context* c)
+{
+    for (auto& kv : c->mb_graphs) (void)hipGraphExecDestroy(kv.second);
+    c->mb_graphs.clear();
+}
+
 int ensure_bwork(pfs_context* c, size_t bytes)
 {
     if (c->bwork_bytes >= bytes) return PFS_OK;
     (void)hipDeviceSynchronize();
+    drop_multi_graphs(c);
     ctx_free(c, c->bwork[0], c->bwork_bytes);
     ctx_free(c, c->bwork[1], c->bwork_bytes);
     c->bwork_bytes = 0;
@@ -1112,6 +1131,227 @@ int idsoft_batch_f32(pfs_context* c, int B, int nsig, const float2* coeffs, floa
     return PFS_OK;
 }
 
+// ---- batches of mixed bandwidth ----------------------------------------------
+
+// One bandwidth group of a mixed batch as the host enqueues it.
+struct MultiRun {
+    Plan* p;
+    int n;             // signals
+    int ld;            // interleaved coefficient row length
+    float2* W;         // the group's part of the work buffers
+    float2* G;
+    float2* const* in_ptrs;   // device tables of the group's signal addresses
+    float2* const* out_ptrs;
+};
+
+int ensure_multi_tables(pfs_context* c, size_t bytes)
+{
+    if (c->mb_bytes >= bytes) return PFS_OK;
+    if (c->mb_ev) HIP_TRY(hipEventSynchronize(c->mb_ev));
+    (void)hipDeviceSynchronize();
+    drop_multi_graphs(c);
+    ctx_free(c, c->mb_dev, c->mb_bytes);
+    if (c->mb_host) (void)hipHostFree(c->mb_host);
+    c->mb_host = nullptr;
+    c->mb_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
+    int rc = ctx_alloc(c, &c->mb_dev, want);
+    if (rc) return rc;
+    if (hipHostMalloc(&c->mb_host, want, hipHostMallocDefault) != hipSuccess) {
+        ctx_free(c, c->mb_dev, want);
+        return fail(PFS_ERR_MEMORY, "mixed batch: pinned table allocation failed");
+    }
+    c->mb_bytes = want;
+    if (!c->mb_ev) HIP_TRY(hipEventCreateWithFlags(&c->mb_ev, hipEventDisableTiming));
+    return PFS_OK;
+}
+
+// The kernels of one mixed batch (no allocation, no host synchronisation, so
+// the sequence can be captured into a graph).
+int multi_enqueue(pfs_context* c, bool fwd, const std::vector<MultiRun>& runs, const BatchMultiGroup* dgroups,
+                  const int4* ditems, int nitems, int nnmax, int grid, hipStream_t st)
+{
+    auto fft = [&](const MultiRun& r, bool forward) -> int {
+        const int N = r.p->t.N, B = r.p->B;
+        const long N3 = (long)N * N * N;
+        if (forward) {
+            LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
+            a.in_ptrs = (const void* const*)r.in_ptrs;
+            a.out_sig = b.in_sig = N3;
+            b.out_sig = N3;
+            HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(nullptr, r.W, a, N, -1, 1.0f, r.p->t.tw32, LINES_ROW_IN_TILE_OUT, r.n, st); }));
+            HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(r.W, r.G, b, N, -1, 1.0f, r.p->t.tw32, LINES_ROW_IN_TILE_OUT, r.n, st); }));
+        } else {
+            LineMap b = map_cols_inv(N), a = map_rows_inv(N);
+            b.in_sig = N3;
+            b.out_sig = a.in_sig = N3;
+            a.out_ptrs = (void* const*)r.out_ptrs;
+            const float scale = (float)(1. / (4. * B * B));
+            HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(r.G, r.W, b, N, +1, 1.0f, r.p->t.tw32, LINES_TILE_IN_ROW_OUT, r.n, st); }));
+            HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(r.W, nullptr, a, N, +1, scale, r.p->t.tw32, LINES_TILE_IN_ROW_OUT, r.n, st); }));
+        }
+        return PFS_OK;
+    };
+    if (fwd) {
+        for (const MultiRun& r : runs) {
+            int rc = fft(r, true);
+            if (rc) return rc;
+        }
+        HIP_TRY(timed(c, "batch_dwt_fwd_multi", st, [&] { return launch_dwt_batch_multi(true, dgroups, ditems, nitems, nnmax, grid, st); }));
+        for (const MultiRun& r : runs)
+            HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(nullptr, r.out_ptrs, r.W, coeff_offset(r.p->B), r.n, r.ld, false, st); }));
+    } else {
+        for (const MultiRun& r : runs) {
+            HIP_TRY(timed(c, "batch_coeff_transpose", st, [&] { return launch_coeff_transpose(nullptr, (float2* const*)r.in_ptrs, r.W, coeff_offset(r.p->B), r.n, r.ld, true, st); }));
+            HIP_TRY(timed(c, "batch_nyquist_zero", st, [&] { return launch_nyquist_zero_f32(r.G, r.p->B, r.n, st); }));
+        }
+        HIP_TRY(timed(c, "batch_dwt_inv_multi", st, [&] { return launch_dwt_batch_multi(false, dgroups, ditems, nitems, nnmax, grid, st); }));
+        for (const MultiRun& r : runs) {
+            int rc = fft(r, false);
+            if (rc) return rc;
+        }
+    }
+    return PFS_OK;
+}
+
+// Forward (grids -> coefficients) or inverse transform of nsig complex64
+// signals of mixed bandwidth, signal i at bws[i] with its input and output at
+// the device addresses in[i] and out[i].  Called with the context lock held.
+int batch_multi(pfs_context* c, bool fwd, int nsig, const int* bws, const void* const* in, void* const* out,
+                hipStream_t st)
+{
+    const char* who = fwd ? "pfs_dsoft_batch_multi_f32_dev" : "pfs_idsoft_batch_multi_f32_dev";
+    if (nsig < 1 || !bws || !in || !out) return fail(PFS_ERR_ARG, "%s: bad arguments", who);
+    // groups by bandwidth, widest first (their pairs are the heaviest work items)
+    std::map<int, std::vector<int>, std::greater<int>> by_b;
+    for (int i = 0; i < nsig; ++i) {
+        if (!in[i] || !out[i]) return fail(PFS_ERR_ARG, "%s: null buffer address of signal %d", who, i);
+        by_b[bws[i]].push_back(i);
+    }
+    std::vector<MultiRun> runs;
+    size_t work = 0;
+    int nnmax = 0;
+    for (auto& kv : by_b) {
+        Plan* p;
+        int rc = get_plan(c, kv.first, &p);
+        if (rc) return rc;
+        if (!batch_supported(p->t.N))
+            return fail(PFS_ERR_UNSUPPORTED, "%s: the batched path supports B = 8, 16, 32, 64, 128 (got %d)", who,
+                        kv.first);
+        rc = ensure_table32(c, *p, c->stream);
+        if (rc) return rc;
+        const size_t N3 = (size_t)p->t.N * p->t.N * p->t.N;
+        runs.push_back(MultiRun{p, (int)kv.second.size(), 0, nullptr, nullptr, nullptr, nullptr});
+        work += kv.second.size() * N3;
+        nnmax = std::max(nnmax, p->t.N);
+    }
+    int rc = ensure_bwork(c, work * sizeof(float2));
+    if (rc) return rc;
+    // table layout: groups | in addresses | out addresses | items
+    const int ng = (int)runs.size();
+    std::vector<int4> items;
+    {
+        struct PairCost { long cost; int g, pi; };
+        std::vector<PairCost> pc;
+        for (int g = 0; g < ng; ++g) {
+            const int B = runs[g].p->B, N = runs[g].p->t.N;
+            int pi = 0;
+            for (int M = 0; M < B; ++M)  // the plan's pair order: M ascending, then M'
+                for (int Mp = 0; Mp <= M; ++Mp, ++pi) pc.push_back(PairCost{(long)(B - M) * N, g, pi});
+        }
+        std::stable_sort(pc.begin(), pc.end(), [](const PairCost& a, const PairCost& b) { return a.cost > b.cost; });
+        std::vector<std::vector<int4>> q(8);
+        for (size_t k = 0; k < pc.size(); ++k) {
+            const int per = MULTI_WPB * MULTI_SPW;
+            const int nsg = (runs[pc[k].g].n + per - 1) / per;
+            for (int sg = 0; sg < nsg; ++sg) q[k % 8].push_back(make_int4(pc[k].g, pc[k].pi, sg, 0));
+        }
+        size_t len = 0;
+        for (auto& v : q) len = std::max(len, v.size());
+        items.assign(8 * len, make_int4(-1, 0, 0, 0));
+        for (int x = 0; x < 8; ++x)
+            for (size_t j = 0; j < q[x].size(); ++j) items[8 * j + x] = q[x][j];
+    }
+    const int nitems = (int)items.size();
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_in = al(ng * sizeof(BatchMultiGroup)), o_out = o_in + al(nsig * sizeof(void*)),
+                 o_it = o_out + al(nsig * sizeof(void*)), bytes = o_it + al(items.size() * sizeof(int4));
+    rc = ensure_multi_tables(c, bytes);
+    if (rc) return rc;
+    HIP_TRY(hipEventSynchronize(c->mb_ev));  // the previous upload from the pinned copy is done
+    char* h = (char*)c->mb_host;
+    char* d = (char*)c->mb_dev;
+    BatchMultiGroup* hg = (BatchMultiGroup*)h;
+    void** hin = (void**)(h + o_in);
+    void** hout = (void**)(h + o_out);
+    float2* W = (float2*)c->bwork[0];
+    float2* G = (float2*)c->bwork[1];
+    size_t off = 0, pos = 0;
+    int g = 0;
+    for (auto& kv : by_b) {
+        MultiRun& r = runs[g];
+        const Plan& p = *r.p;
+        const long N3 = (long)p.t.N * p.t.N * p.t.N, count = coeff_offset(p.B);
+        r.ld = interleave_ld(r.n, r.n, count, N3);
+        r.W = W + off;
+        r.G = G + off;
+        r.in_ptrs = (float2* const*)(d + o_in) + pos;
+        r.out_ptrs = (float2* const*)(d + o_out) + pos;
+        for (int i : kv.second) {
+            hin[pos] = const_cast<void*>(in[i]);
+            hout[pos] = out[i];
+            ++pos;
+        }
+        BatchMultiGroup& bg = hg[g];
+        bg.t = p.t;
+        bg.roff = p.bt.roff;
+        bg.wts32 = p.bt.wts32;
+        bg.tab = fwd ? p.bt.tf : p.bt.ti;
+        bg.in = fwd ? (const void*)r.G : (const void*)r.W;
+        bg.out = fwd ? r.W : r.G;
+        bg.nsig = r.n;
+        bg.cdist = 1;
+        bg.cstride = r.ld;
+        bg.norm = (float)(fwd ? fwd_norm(p.B) : inv_norm(p.B));
+        off += (size_t)r.n * N3;
+        ++g;
+    }
+    memcpy(h + o_it, items.data(), items.size() * sizeof(int4));
+    HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(c->mb_ev, st));
+    const BatchMultiGroup* dgroups = (const BatchMultiGroup*)d;
+    const int4* ditems = (const int4*)(d + o_it);
+    const int grid = std::min(nitems, 1024);  // persistent: a multiple of 8 (XCD residues kept)
+
+    if (!c->batch_graphs) return multi_enqueue(c, fwd, runs, dgroups, ditems, nitems, nnmax, grid, st);
+    // the kernels' arguments depend only on the composition (the addresses are
+    // in the device tables), so one captured graph per composition is replayed
+    std::string key = fwd ? "f" : "i";
+    for (const MultiRun& r : runs) key += "|" + std::to_string(r.p->B) + "x" + std::to_string(r.n);
+    auto it = c->mb_graphs.find(key);
+    if (it == c->mb_graphs.end()) {
+        if (!c->mb_cap) HIP_TRY(hipStreamCreateWithFlags(&c->mb_cap, hipStreamNonBlocking));
+        const bool timing = c->timing;
+        c->timing = false;  // no timing events inside a graph
+        HIP_TRY(hipStreamBeginCapture(c->mb_cap, hipStreamCaptureModeThreadLocal));
+        rc = multi_enqueue(c, fwd, runs, dgroups, ditems, nitems, nnmax, grid, c->mb_cap);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->mb_cap, &graph);
+        c->timing = timing;
+        if (rc || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc ? rc : fail(PFS_ERR_DEVICE, "%s: graph capture failed: %s", who, hipGetErrorString(ec));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(PFS_ERR_DEVICE, "%s: graph instantiation failed: %s", who, hipGetErrorString(ei));
+        it = c->mb_graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, st));
+    return PFS_OK;
+}
+
 std::mutex g_seam_mu;
 pfs_context* g_seam_ctx = nullptr;
 
@@ -1200,6 +1440,12 @@ int pfs_ctx_destroy(pfs_context* c)
         }
         ctx_free(c, c->bwork[0], c->bwork_bytes);
         ctx_free(c, c->bwork[1], c->bwork_bytes);
+        for (auto& kv : c->mb_graphs) (void)hipGraphExecDestroy(kv.second);
+        c->mb_graphs.clear();
+        ctx_free(c, c->mb_dev, c->mb_bytes);
+        if (c->mb_host) (void)hipHostFree(c->mb_host);
+        if (c->mb_ev) (void)hipEventDestroy(c->mb_ev);
+        if (c->mb_cap) (void)hipStreamDestroy(c->mb_cap);
         c->plans.clear();
         ctx_free(c, c->work[0], c->work_bytes);
         ctx_free(c, c->work[1], c->work_bytes);
@@ -1452,6 +1698,24 @@ int pfs_idsoft_batch_f32_dev(pfs_context* c, int B, int nsig, const void* coeffs
     return idsoft_batch_f32(c, B, nsig, (const float2*)coeffs_dev, (float2*)grids_dev, pick(c, stream));
 }
 
+int pfs_dsoft_batch_multi_f32_dev(pfs_context* c, int nsig, const int* bandwidths, const void* const* grids_dev,
+                                  void* const* coeffs_dev, void* stream)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return batch_multi(c, true, nsig, bandwidths, grids_dev, coeffs_dev, pick(c, stream));
+}
+
+int pfs_idsoft_batch_multi_f32_dev(pfs_context* c, int nsig, const int* bandwidths, const void* const* coeffs_dev,
+                                   void* const* grids_dev, void* stream)
+{
+    if (!c) return fail(PFS_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return batch_multi(c, false, nsig, bandwidths, coeffs_dev, grids_dev, pick(c, stream));
+}
+
 int pfs_tune_set(pfs_context* c, const char* key, long value)
 {
     if (!c || !key) return fail(PFS_ERR_ARG, "pfs_tune_set: null argument");
@@ -1469,6 +1733,11 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
     if (!strcmp(key, "batch_interleave")) {
         if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "batch_interleave must be 0 or 1");
         c->batch_interleave = value;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_graphs")) {
+        if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "batch_graphs must be 0 or 1");
+        c->batch_graphs = value;
         return PFS_OK;
     }
     if (!strcmp(key, "batch_precision")) {
@@ -1542,6 +1811,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_precision")) {
         *value = c->batch_precision;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_graphs")) {
+        *value = c->batch_graphs;
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_inv_kernel")) {

@@ -226,6 +226,30 @@ hipError_t launch_dwt_fwd_batch_f32(const PlanTables& t, const BatchTables& bt, 
 hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, const float2* coeffs, float2* G,
                                     int nsig, long cdist, long cstride, float norm, BatchKernelShape sh,
                                     hipStream_t st, int prec = 0);
+// One bandwidth group of a mixed-bandwidth batched contraction
+// (launch_dwt_batch_multi): the group's plan and FP32 tile table (Tf forward,
+// Ti inverse), its input and output (forward: pencils -> coefficients,
+// inverse: coefficients -> pencils), and the coefficient layout
+// s * cdist + idx * cstride as in launch_dwt_fwd_batch_f32.  Lives in device
+// memory: the kernel reads the group of each work item from there.
+struct BatchMultiGroup {
+    PlanTables t;
+    const long* roff;
+    const float* wts32;
+    const float* tab;
+    const void* in;
+    float2* out;
+    int nsig;
+    long cdist, cstride;
+    float norm;
+};
+// Work shape of the mixed-bandwidth contraction: 4 waves x 2 signals per item.
+constexpr int MULTI_WPB = 4, MULTI_SPW = 2;
+// One persistent launch over work items int4{group, pair index, signal group,
+// 0} (group < 0: padding), grid workgroups walking items b, b + grid, ...;
+// nnmax: largest 2B among the groups.
+hipError_t launch_dwt_batch_multi(bool fwd, const BatchMultiGroup* groups, const int4* items, int nitems, int nnmax,
+                                  int grid, hipStream_t st);
 // Between the per-signal layout (per_signal + s * count, or sptr[s] when sptr
 // is set) and the interleaved layout [idx][ld], in the direction given.
 hipError_t launch_coeff_transpose(float2* per_signal, float2* const* sptr, float2* interleaved, long count, int n,

@@ -215,3 +215,65 @@ def test_batch_precision_key_rejects_other_values(ctx, torch):
     with pytest.raises(P.PfsError):
         ctx.tune("batch_precision", 4)
     assert ctx.tune_get("batch_precision") == 0
+
+
+MIX = [64, 16, 128, 32, 16, 64, 32, 16, 128, 32, 16]  # the serving bandwidths, shuffled
+
+
+def _mix_inputs(torch, seed0):
+    cs = [O.seeded_coeffs(B, seed0 + i) for i, B in enumerate(MIX)]
+    cd = [torch.from_numpy(c.astype(np.complex64)).cuda() for c in cs]
+    gd = [torch.full(((2 * B) ** 3,), float("nan"), dtype=torch.complex64, device="cuda") for B in MIX]
+    return cs, cd, gd
+
+
+def test_mixed_bandwidth_batch_against_oracle_and_single_bandwidth_path(ctx, torch):
+    """pfs_{id,d}soft_batch_multi_f32_dev: every signal of a {16, 32, 64, 128}
+    mix within 1e-6 of the FP64 oracle per direction, and bit-equal to the
+    single-bandwidth batched entry points on the same inputs."""
+    cs, cd, gd = _mix_inputs(torch, 5100)
+    s = torch.cuda.current_stream()
+    ctx.idsoft_batch_multi(MIX, cd, gd, stream=s)
+    back = [torch.full_like(c, float("nan")) for c in cd]
+    ctx.dsoft_batch_multi(MIX, gd, back, stream=s)
+    torch.cuda.synchronize()
+    for i, B in enumerate(MIX):
+        g = gd[i].cpu().numpy().astype(np.complex128)
+        e_inv = rel(g, O.idsoft(B, cs[i]))
+        e_fwd = rel(back[i].cpu().numpy().astype(np.complex128), O.dsoft(B, g))
+        assert e_inv < TOL and e_fwd < TOL, (i, B, e_inv, e_fwd)
+    # the single-bandwidth path, one call per bandwidth, gives the same bits
+    for B in sorted(set(MIX)):
+        idx = [i for i, b in enumerate(MIX) if b == B]
+        n, N3 = O.coeff_count(B), (2 * B) ** 3
+        c_cat = torch.cat([cd[i] for i in idx])
+        g_cat = torch.empty(len(idx) * N3, dtype=torch.complex64, device="cuda")
+        ctx.idsoft_batch_f32(B, len(idx), c_cat, g_cat, stream=s)
+        r_cat = torch.empty_like(c_cat)
+        ctx.dsoft_batch_f32(B, len(idx), torch.cat([gd[i] for i in idx]), r_cat, stream=s)
+        torch.cuda.synchronize()
+        for j, i in enumerate(idx):
+            assert torch.equal(g_cat[j * N3:(j + 1) * N3], gd[i]), ("inverse", B, i)
+            assert torch.equal(r_cat[j * n:(j + 1) * n], back[i]), ("forward", B, i)
+
+
+def test_mixed_bandwidth_graph_replays_match_direct_launches(ctx, torch):
+    """The captured pipeline of a composition, replayed on new inputs with the
+    program's own work and synchronisations in between, equals the directly
+    launched sequence on every replay."""
+    s = torch.cuda.current_stream()
+    ref = None
+    for rnd in range(4):
+        cs, cd, gd = _mix_inputs(torch, 6200 + rnd)
+        scratch = torch.randn(1 << 20, device="cuda")  # the program's own work in between
+        scratch.mul_(2.0).sum().item()
+        ctx.tune("batch_graphs", 1)
+        ctx.idsoft_batch_multi(MIX, cd, gd, stream=s)
+        torch.cuda.synchronize()
+        direct = [torch.full_like(g, float("nan")) for g in gd]
+        ctx.tune("batch_graphs", 0)
+        ctx.idsoft_batch_multi(MIX, cd, direct, stream=s)
+        torch.cuda.synchronize()
+        ctx.tune("batch_graphs", 1)
+        for g, d in zip(gd, direct):
+            assert torch.equal(g, d), rnd
