@@ -213,7 +213,13 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *   "batch_interleave"  1 (default): batched contraction on signal-interleaved
  *                       coefficients with transposes; 0: on the caller's layout.
  *   "dwt_variant"       work shape of the FP64 DWT kernels for 2B <= 256
- *                       (0 default, 1..3 alternatives; results equal).
+ *                       (0 default, 1..6 alternatives; results agree to
+ *                       rounding).  Forward at 2B = 128, 256: 0 the
+ *                       node-blocked kernel (one 64-node block at a time,
+ *                       all degree chunks in accumulators), 4 / 5 the same
+ *                       held to three / two waves per SIMD, 6 and 1..3 the
+ *                       chain-parallel kernel (all nodes at once) in its
+ *                       shapes.
  *   "dwt_inv_kernel"    inverse FP64 DWT kernel: 7 (default) the vector-ALU
  *                       kernel with its row broadcasts folded into the FMAs,
  *                       0 the MFMA kernel; 1..3 / 5..7 select vector-ALU work

@@ -1759,7 +1759,7 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_variant")) {
-        if (value < 0 || value > 3) return fail(PFS_ERR_ARG, "dwt_variant must be in 0..3");
+        if (value < 0 || value > 6) return fail(PFS_ERR_ARG, "dwt_variant must be in 0..6");
         c->dwt_variant = value;
         return PFS_OK;
     }

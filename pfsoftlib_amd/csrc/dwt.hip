@@ -209,17 +209,27 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__
 // 16 rows an MFMA operand read touches fall in distinct banks).
 __host__ __device__ constexpr int tile_stride(int nc) { return 64 * nc + 1; }
 
+// Row stride of the forward kernels' tile.  gfx950 serves a 64-bit LDS read
+// 16 lanes at a time over 32 banks (measured, SQ_LDS_BANK_CONFLICT: a 2-word
+// row shift read conflict-free, a 4-word shift 2-way).  The forward's
+// A-operand read has lanes (row r, node kk) at words 2 r TS + 2 kk, a quarter
+// being one kk and r = 0..15: with TS = 64 nc + 1 the 16 rows sit 2 words
+// apart (mod 32) and cover all 32 banks once.
+__host__ __device__ constexpr int fwd_tile_stride(int nc) { return 64 * nc + 1; }
+
 // Pencil staging of the forward preamble: 8 cases x 64 nodes x (re, im) for one
-// chain, case rows padded to 136 doubles (272 words = 16 mod 64) so the
-// operand-layout reads spread over all banks.
-constexpr int FSTG_CASE = 136;
+// chain, case rows 130 doubles apart (260 words = 4 mod 32): the
+// operand-layout read of a 16-lane quarter (case j / 2, re/im j % 2) then
+// covers 8 case slots x 4 words = all 32 banks once.  (At 136 doubles, 16 mod
+// 32, four cases shared each bank group: 6.3 M conflict cycles per launch.)
+constexpr int FSTG_CASE = 130;
 constexpr int FSTG = 8 * FSTG_CASE;
 
 // Doubles of LDS per wave of the forward kernel: its 16-row tile, which during
 // the preamble holds the pencil staging.
 __host__ __device__ constexpr int fwd_wave_region(int nh)
 {
-    return 16 * tile_stride(nh) > FSTG ? 16 * tile_stride(nh) : FSTG;
+    return 16 * fwd_tile_stride(nh) > FSTG ? 16 * fwd_tile_stride(nh) : FSTG;
 }
 
 // Doubles of LDS per wave of the inverse kernel: its 16-row tile, which after
@@ -258,7 +268,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
                                                           double2* __restrict__ coeffs, double norm)
 {
     extern __shared__ __align__(16) double sm[];
-    constexpr int TS = tile_stride(NH);
+    constexpr int TS = fwd_tile_stride(NH);
     constexpr int WREG = fwd_wave_region(NH);
     const int B = t.B, N = t.N;
     const int2 pr = t.xpairs[blockIdx.x];
@@ -390,6 +400,133 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
             }
         }
     }
+}
+
+// Forward DWT, node blocks outermost (2B = 64 NB): one wave per pair walks
+// the pair's NB blocks of 64 beta nodes, and for each block runs the
+// recurrence over all of the pair's NCHK 16-degree chunks, accumulating every
+// chunk's 16 x 16 output block in its own MFMA accumulator.  Only one block's
+// operands (16 doubles per lane) are live, so the wave holds about half the
+// registers of the chain-parallel kernel, and the next block's pencils and
+// node tables are loaded while the current block's chunks compute.  The LDS
+// region per wave is the 16 x 65 recurrence tile, which also stages each
+// block's pencils into the MFMA operand layout, plus the step coefficients.
+template <int NB, int NCHK>
+__device__ __forceinline__ void fwd_nb_pair(const PlanTables& t, const double2* __restrict__ G,
+                                            double2* __restrict__ coeffs, double norm, int M, int Mp, double* sm)
+{
+    constexpr int TS = fwd_tile_stride(1);
+    constexpr int WREG = 16 * TS > FSTG ? 16 * TS : FSTG;
+    constexpr int RP = NCHK * 16;
+    double* tile = sm;
+    double* cf = sm + WREG;
+    const int B = t.B, N = t.N, R = B - M;
+    const int lane = threadIdx.x & 63, j = lane & 15, kk = lane >> 4;
+    const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
+    long cvec[8];
+    int cflip = 0, cact = 0;
+#pragma unroll
+    for (int cs = 0; cs < 8; ++cs) {
+        const CaseInfo ci = case_info(cs, M, Mp, N);
+        cvec[cs] = ci.vec;
+        cflip |= ci.flip << cs;
+        cact |= ci.active << cs;
+    }
+    // block b of case cs: node n = 64 b + lane, reversed for the d(pi - beta) cases
+    auto load_block = [&](int b, double2 (&raw)[8]) {
+        const int n = 64 * b + lane;
+#pragma unroll
+        for (int cs = 0; cs < 8; ++cs) raw[cs] = load_nt(G + cvec[cs] + (((cflip >> cs) & 1) ? N - 1 - n : n));
+    };
+    double2 raw[8];
+    load_block(0, raw);
+    ChainTables<1> ct;
+    load_chain_tables<1>(ct, t, lane, true);
+    const double nf = t.normfac[M * B + Mp];
+    for (int r = lane; r < RP; r += 64) recurrence_coeffs(M, Mp, r, R, cf, RP);
+
+    v4d acc[NCHK];
+#pragma unroll
+    for (int c = 0; c < NCHK; ++c) acc[c] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int b = 0; b < NB; ++b) {
+        wave_sync();  // step coefficients written / the previous block's tile reads done
+#pragma unroll
+        for (int cs = 0; cs < 8; ++cs) {
+            const bool on = (cact >> cs) & 1;
+            double2 v;
+            v.x = on ? raw[cs].x : 0.0;
+            v.y = on ? raw[cs].y : 0.0;
+            *reinterpret_cast<double2*>(tile + cs * FSTG_CASE + 2 * lane) = v;
+        }
+        wave_sync();
+        // B operands: lane (j + 16 kk) holds S[64 b + 4 m + kk][j], j = 2 case + re/im
+        double sreg[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) sreg[m] = tile[(j >> 1) * FSTG_CASE + 2 * (4 * m + kk) + (j & 1)];
+        Chains<1> ch;
+        init_chains<1>(ch, ct, nf, M, Mp, 64 * b + lane, N);
+        // the next block's loads stay in flight across this block's chunks
+        if (b + 1 < NB) {
+            load_block(b + 1, raw);
+            load_chain_tables<1>(ct, t, 64 * (b + 1) + lane, true);
+        }
+        wave_sync();  // operand reads done before the recurrence writes the tile
+#pragma unroll
+        for (int chk = 0; chk < NCHK; ++chk) {
+            run_chains<1, 0, 1>(ch, tile, TS, lane, cf, RP, chk * 16);
+            wave_sync();
+            v4d a1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const double a = tile[(lane & 15) * TS + 4 * m + kk];
+                if (m & 1) a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[m], a1, 0, 0, 0);
+                else acc[chk] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sreg[m], acc[chk], 0, 0, 0);
+            }
+            acc[chk] += a1;
+            wave_sync();  // tile reads done before the next chunk's recurrence writes
+        }
+    }
+    if (!lc.active) return;
+    const int orow = lane >> 4;
+#pragma unroll
+    for (int chk = 0; chk < NCHK; ++chk)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = chk * 16 + 4 * v + orow;
+            if (r < R) {
+                const double sg = (r & 1) ? lc.sgn_odd : lc.sgn;
+                reinterpret_cast<double*>(coeffs + coeff_index32(M + r, lc.Mc, lc.Mpc))[j & 1] = norm * (sg * acc[chk][v]);
+            }
+        }
+}
+
+template <int NB, int NCHK>
+__device__ __forceinline__ void fwd_nb_dispatch(int nch, const PlanTables& t, const double2* __restrict__ G,
+                                                double2* __restrict__ coeffs, double norm, int M, int Mp, double* sm)
+{
+    if constexpr (NCHK <= 2 * NB) {  // B / 16 = 2 NB chunks at most
+        if (nch == NCHK) fwd_nb_pair<NB, NCHK>(t, G, coeffs, norm, M, Mp, sm);
+        else fwd_nb_dispatch<NB, NCHK + 1>(nch, t, G, coeffs, norm, M, Mp, sm);
+    }
+}
+
+template <int NB, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dwt_fwd_f64_nb(PlanTables t, const double2* __restrict__ G,
+                                                       double2* __restrict__ coeffs, double norm)
+{
+    extern __shared__ __align__(16) double sm[];
+    const int2 pr = t.xpairs[blockIdx.x];
+    const int nch = (t.B - pr.x + 15) >> 4;
+    fwd_nb_dispatch<NB, 1>(nch, t, G, coeffs, norm, pr.x, pr.y, sm);
+}
+
+// LDS bytes of the node-blocked forward: tile / staging region + step coefficients.
+static size_t fwd_nb_smem(int B)
+{
+    const int ts = fwd_tile_stride(1);
+    const int wreg = 16 * ts > FSTG ? 16 * ts : FSTG;
+    return (size_t)(wreg + 3 * (((B + 15) / 16) * 16)) * sizeof(double);
 }
 
 // G[p][q][k] <- grid(row p, col q, slice k) for p == B or q == B.
@@ -775,6 +912,22 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
 {
     if (t.N > 1024) return hipErrorInvalidValue;
     if (t.N > 256) return launch_dwt_fwd_f64_large(t, G, coeffs, norm, st);
+    // Default at 2B = 128 and 256: the node-blocked kernel, held to three waves
+    // per SIMD at 2B = 256 (168 VGPRs, 52 bytes of spills; measured 101.5 us
+    // against 105.0 us for the chain-parallel kernel and 109.5 us at two waves)
+    // and two at 2B = 128 (16.8 us against 18.0 and 18.6 us).  Variants 4 and
+    // 5 force three or two waves; 6 is the chain-parallel kernel.
+    if ((variant == 0 || variant == 4 || variant == 5) && (t.N == 128 || t.N == 256)) {
+        const size_t smem = fwd_nb_smem(t.B);
+        const dim3 grid(t.npairs), block(64);
+        const bool three = variant == 4 || (variant == 0 && t.N == 256);
+        if (three)
+            return t.N == 128 ? pfs::launch((k_dwt_fwd_f64_nb<2, 3>), grid, block, smem, st, t, G, coeffs, norm)
+                              : pfs::launch((k_dwt_fwd_f64_nb<4, 3>), grid, block, smem, st, t, G, coeffs, norm);
+        return t.N == 128 ? pfs::launch((k_dwt_fwd_f64_nb<2, 2>), grid, block, smem, st, t, G, coeffs, norm)
+                          : pfs::launch((k_dwt_fwd_f64_nb<4, 2>), grid, block, smem, st, t, G, coeffs, norm);
+    }
+    if (variant >= 4) variant = 0;  // the chain-parallel kernel's default shape
     const DwtShape s = dwt_shape(t.N, true, variant);
     const size_t smem = dwt_smem(t.B, t.N, true, variant);
     if (smem > lds_limit()) return hipErrorInvalidConfiguration;

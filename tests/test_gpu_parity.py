@@ -158,11 +158,12 @@ def test_dsoft_b128_matches_oracle(ctx, b128):
     assert np.abs(got.real - c.real).max() <= 1e-11 and np.abs(got.imag - c.imag).max() <= 1e-11
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
     # every work shape of the FP64 DWT kernels (chains per lane x waves per
-    # pair), at B = 128 and at grids that leave part of the nodes a shape
-    # covers empty (2B = 64, 80, 128)
+    # pair; 4 / 5 the node-blocked forward at three / two waves per SIMD, 6 the
+    # chain-parallel forward's default shape), at B = 128 and at grids that
+    # leave part of the nodes a shape covers empty (2B = 64, 80, 128)
     B, c, g_ref, c_ref = b128
     ctx.tune("dwt_variant", variant)
     try:
