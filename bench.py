@@ -585,15 +585,21 @@ def dwt_accuracy(ctx, np, max_bw, runs, seed):
     return out
 
 
-def copy_calibration(torch, steps, mib):
-    """A plain elementwise copy of known size, for scaling PMC byte counters."""
-    n = mib * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
+def copy_calibration(torch, P, steps, mib):
+    """Copies of known size for scaling PMC byte counters: the library's
+    16-byte copy kernel (the FFT passes' access width) with ordinary and with
+    non-temporal loads and stores (kernels k_copy16<false> / <true>)."""
+    ctx = P.Context(0)
+    n = mib * (1 << 20)
+    a = torch.ones(n // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
-    for _ in range(steps):
-        torch.mul(a, 1.0, out=b)
+    for nt in (0, 1):
+        for _ in range(steps):
+            ctx.copy_dev(b, a, n, nontemporal=nt)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration": {"bytes_read": n * 8, "bytes_written": n * 8, "launches": steps}}))
+    print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches_per_kernel": steps,
+                                           "kernels": ["k_copy16<false>", "k_copy16<true>"]}}))
+    ctx.close()
 
 
 def main():
@@ -643,7 +649,7 @@ def main():
     import pfsoftlib_amd as P
 
     if args.copy_calibration:
-        copy_calibration(torch, max(args.steps, 1), args.copy_calibration)
+        copy_calibration(torch, P, max(args.steps, 1), args.copy_calibration)
         return
 
     B = args.bandwidth

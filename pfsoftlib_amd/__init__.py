@@ -112,6 +112,7 @@ def lib():
             "pfs_memcpy_h2d": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_memcpy_d2h": (ip, [vp, vp, vp, ctypes.c_size_t]),
             "pfs_synchronize": (ip, [vp, vp]),
+            "pfs_copy_dev": (ip, [vp, vp, vp, ctypes.c_size_t, ip, vp]),
             "pfs_dsoft_batch_multi_f32_dev": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(vp),
                                                   ctypes.POINTER(vp), vp]),
             "pfs_idsoft_batch_multi_f32_dev": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(vp),
@@ -351,6 +352,11 @@ class Context:
     def layer_wise_dft2_dev(self, n, lays, arr_dev, sign, scale=1.0, stream=None):
         ptr = _devbuf(arr_dev, "arr", "c128", max(int(n), 0) ** 2 * max(int(lays), 0), self.device)
         _check(lib().pfs_layer_wise_dft2_dev(self._h, n, lays, ptr, sign, scale, _stream_handle(stream)))
+
+    def copy_dev(self, dst, src, nbytes, nontemporal=False, stream=None):
+        """dst <- src (device addresses or tensors), 16-byte accesses, optionally non-temporal."""
+        _check(lib().pfs_copy_dev(self._h, _devptr(dst), _devptr(src), nbytes, 1 if nontemporal else 0,
+                                  _stream_handle(stream)))
 
     def synchronize(self, stream=None):
         _check(lib().pfs_synchronize(self._h, _stream_handle(stream)))

@@ -1384,6 +1384,20 @@ int seam_dft2(int cols, int rows, int lays, double* arr, int sign, double scale_
 }  // namespace
 
 // ===========================================================================
+// dst <- src in 16-byte elements, ordinary or non-temporal loads and stores:
+// the access widths and cache hints of the FFT passes.
+template <bool NT>
+__global__ void k_copy16(const double2* __restrict__ src, double2* __restrict__ dst, long n)
+{
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const v2d* s = reinterpret_cast<const v2d*>(src);
+    v2d* d = reinterpret_cast<v2d*>(dst);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
+        else d[i] = s[i];
+    }
+}
+
 extern "C" {
 
 const char* pfs_last_error(void) { return g_last_error.c_str(); }
@@ -1944,6 +1958,22 @@ int pfs_memcpy_d2h(pfs_context* c, void* dst, const void* src, size_t bytes)
     if (!c || !dst || !src) return fail(PFS_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PFS_OK;
+}
+
+int pfs_copy_dev(pfs_context* c, void* dst_dev, const void* src_dev, size_t bytes, int nontemporal, void* stream)
+{
+    if (!c || !dst_dev || !src_dev || (bytes & 15) || ((uintptr_t)dst_dev & 15) || ((uintptr_t)src_dev & 15))
+        return fail(PFS_ERR_ARG, "pfs_copy_dev: null argument, or a size or address not a multiple of 16 bytes");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    const long n = (long)(bytes / 16);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+    if (n == 0) return PFS_OK;
+    if (nontemporal)
+        HIP_TRY(timed(c, "copy16_nt", pick(c, stream), [&] { return pfs::launch(k_copy16<true>, dim3(blocks), dim3(256), 0, pick(c, stream), (const double2*)src_dev, (double2*)dst_dev, n); }));
+    else
+        HIP_TRY(timed(c, "copy16", pick(c, stream), [&] { return pfs::launch(k_copy16<false>, dim3(blocks), dim3(256), 0, pick(c, stream), (const double2*)src_dev, (double2*)dst_dev, n); }));
     return PFS_OK;
 }
 
