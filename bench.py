@@ -25,6 +25,7 @@ import time
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 FP32_MFMA_PEAK_TFS = 157.3  # MI355X dense FP32 matrix, published
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 matrix and FP64 vector, published
 
 
 def algorithmic_bytes_per_step(kernel, B, e=16):
@@ -55,6 +56,35 @@ def measured_traffic(B, kernel):
             return json.load(f)["kernels"][kernel]["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def measured_pipes(B):
+    """Per-kernel SQ instruction counters of this workload from the committed
+    rocprofv3 summary (profiles/pmc_pipes_f64_B<B>.json), or {}."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_pipes_f64_B%d.json" % B)
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"]
+    except (OSError, KeyError, ValueError):
+        return {}
+
+
+def pipe_report(B, kernels):
+    """For each kernel with committed SQ counters: the SIMD issue floor (VALU
+    and MFMA cycles the counters give, spread over every SIMD at the peak
+    clock) against the live average launch time, and the FP64 rate the
+    counted MFMA and FMA work reaches at that time."""
+    res = {}
+    for name, c in measured_pipes(B).items():
+        if name not in kernels:
+            continue
+        us = kernels[name]["avg_launch_us"]
+        flop = c["mfma_f64_flop"] + c["valu_f64_fma_flop"]
+        res[name] = {"issue_floor_us": c["issue_floor_us"], "avg_launch_us": us,
+                     "issue_frac": round(c["issue_floor_us"] / us, 4),
+                     "fp64_TFLOPs": round(flop / (us * 1e-6) / 1e12, 2),
+                     "fp64_frac": round(flop / (us * 1e-6) / 1e12 / FP64_PEAK_TFS, 4)}
+    return res
 
 
 def cpu_model():
@@ -768,7 +798,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": measured_traffic(B, dom),
                      "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"]),
-                     "avg_launch_us": dk["avg_launch_us"]},
+                     "avg_launch_us": dk["avg_launch_us"],
+                     "issue": pipe_report(B, {dom: dk}).get(dom)},
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         # kernel time per direction (sum of its kernels' average durations)
         "idsoft_ms": {"kernels_avg": kernel_ms(["dwt_inv", "nyquist_gather", "fft_inv_cols", "fft_inv_rows"])},
@@ -777,6 +808,7 @@ def main():
         "roundtrip_rel_l2": err,
         "kernels": kernels,
         "fft_stage": fft_stage(),
+        "pipes": pipe_report(B, kernels),
         "kernels_note": "per-kernel breakdown from a separate pass of %d round trips with every kernel timed; "
                         "the timed region times only %s (the roofline kernel)" % (nbreak, dom),
     }
