@@ -220,12 +220,15 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *                       held to three / two waves per SIMD, 6 and 1..3 the
  *                       chain-parallel kernel (all nodes at once) in its
  *                       shapes.
- *   "dwt_inv_kernel"    inverse FP64 DWT kernel: 7 (default) the vector-ALU
- *                       kernel with its row broadcasts folded into the FMAs,
- *                       0 the MFMA kernel; 1..3 / 5..7 select vector-ALU work
+ *   "dwt_inv_kernel"    inverse FP64 DWT kernel: 15 (default) the staged
+ *                       vector-ALU kernel (coefficients staged in LDS a chunk
+ *                       of 16 degrees at a time), 0 the MFMA kernel; 1..3 /
+ *                       5..7 select the unstaged vector-ALU kernel's work
  *                       shapes (2 chains x 2 waves over the nodes, 4 x 2 over
- *                       the right-hand sides, 4 x 1; 5..7 with the folded
- *                       broadcasts; results agree to rounding).
+ *                       the right-hand sides, 4 x 1; 5..7 with the row
+ *                       broadcasts folded into the FMAs), 13 / 15 the staged
+ *                       kernel in shapes 1 / 3 (bit-identical to 5 / 7);
+ *                       results agree to rounding.
  *   "dwt_pair_order"    pair launch order of the FP64 DWT kernels, bit 0 the
  *                       forward's, bit 1 the inverse's: 0 = every XCD takes a
  *                       contiguous M' range of each M (default, both), 1 = runs

@@ -280,10 +280,12 @@ struct pfs_context {
     // inverse DWT kernel: 0 MFMA; vector ALU with 2 chains x 2 waves over the
     // nodes, 4 chains x 2 waves over the right-hand sides, or 4 x 1 per pair
     // (2B <= 256; 4 x 2 at 2B = 512): 1..3 with separate broadcast moves,
-    // 5..7 with the broadcasts folded into the FMAs.
+    // 5..7 with the broadcasts folded into the FMAs; 13 and 15 the staged
+    // kernel (k_dwt_inv_f64_staged) in shapes 1 and 3.
     // Measured: 7 takes B = 128 from 120 to 109-113 us, B = 256 from 1.49 to
-    // 1.30 ms.
-    long dwt_inv_kernel = 7;
+    // 1.30 ms; 15 takes B = 128 from 110.2-111.5 to about 97 us (its FP64 FMA
+    // count is 7's, its other VALU and SALU work per degree much smaller).
+    long dwt_inv_kernel = 15;
     // pair launch order of the FP64 DWT kernels (Plan::xorder): bit 0 the
     // forward's, bit 1 the inverse's.  Measured at B = 128: the per-XCD M'
     // ranges halve the inverse's HBM fetch (183 -> 96 MB) and cut the forward's
@@ -395,7 +397,10 @@ int check_bandwidth(int B)
 // (l, +-M', .) are neighbours for consecutive M' and (l, +-M, .) for
 // consecutive M: one L2 then serves the coefficient lines of all eight cases
 // of neighbouring pairs.  Which XCD gets a range's extra pair rotates with M,
-// so the eight queues stay within one pair of each other.
+// so the eight queues stay within one pair of each other.  (Alternating each
+// queue's heaviest and lightest pairs instead, so that pairs that mostly
+// compute run beside pairs that mostly store, measured slower at B = 128:
+// inverse 103 -> 128 us, forward 104 -> 137 us.)
 std::vector<int2> xcd_pair_order(int B)
 {
     constexpr int X = 8;
@@ -1762,8 +1767,8 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         return PFS_OK;
     }
     if (!strcmp(key, "dwt_inv_kernel")) {
-        if (value < 0 || value > 7 || value == 4)
-            return fail(PFS_ERR_ARG, "dwt_inv_kernel must be 0 (MFMA), 1..3 or 5..7 (vector ALU)");
+        if (value < 0 || value > 15 || value == 4 || (value >= 8 && value != 13 && value != 15))
+            return fail(PFS_ERR_ARG, "dwt_inv_kernel must be 0 (MFMA), 1..3 or 5..7 (vector ALU), 13 or 15 (staged)");
         c->dwt_inv_kernel = value;
         return PFS_OK;
     }

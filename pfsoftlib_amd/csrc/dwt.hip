@@ -84,11 +84,11 @@ __device__ __forceinline__ double ipow(double x, int e)
 }
 
 // Recurrence coefficients for the step from degree l = M + r to l + 1
-// (fn_dwt.hpp:167-192), FP64.  Stored as (f1, f1*f2, c1); zero past the
-// last row so the padded rows stay 0.
-__device__ __forceinline__ void recurrence_coeffs(int M, int Mp, int r, int R, double* cf, int stride)
+// (fn_dwt.hpp:167-192), FP64: (f1, f1*f2, c1), zero from the last row on so
+// the padded rows stay 0.
+__device__ __forceinline__ void recurrence_step(int M, int Mp, int r, int R, double& f1, double& f1f2, double& c1)
 {
-    double f1 = 0, f1f2 = 0, c1 = 0;
+    f1 = 0, f1f2 = 0, c1 = 0;
     if (r < R - 1) {
         double idx = M + r;
         double nrm = sqrt((2.0 * idx + 3.0) / (2.0 * idx + 1.0));
@@ -104,6 +104,13 @@ __device__ __forceinline__ void recurrence_coeffs(int M, int Mp, int r, int R, d
         }
         f1f2 = f1 * f2;
     }
+}
+
+// The same, stored as three rows of `stride` values.
+__device__ __forceinline__ void recurrence_coeffs(int M, int Mp, int r, int R, double* cf, int stride)
+{
+    double f1, f1f2, c1;
+    recurrence_step(M, Mp, r, R, f1, f1f2, c1);
     cf[r] = f1;
     cf[stride + r] = f1f2;
     cf[2 * stride + r] = c1;
@@ -871,6 +878,155 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
     }
 }
 
+// The inverse of k_dwt_inv_f64_valu with its per-degree overhead moved out of
+// the degree loop:
+//   - the coefficients of a chunk's 16 degrees (16 x 16 values, norm and sign
+//     applied) are loaded one chunk ahead, four per lane, and staged in the
+//     wave's LDS slice; each degree reads its 16 values back with one
+//     ds_read_b64 (16-lane rows read the same 128 bytes), so the 16 index
+//     computations per lane and chunk become 4, and no ring of 16 registers
+//     is held;
+//   - the step coefficients sit interleaved (f1, f1 f2, c1, 0) per degree and
+//     every lane reads them at one address (an LDS broadcast) instead of a
+//     DPP row broadcast per value;
+//   - full chunks run unrolled with no bounds test; the last R % 16 degrees
+//     of a pair run one at a time (no padded degree is computed).
+// Results equal k_dwt_inv_f64_valu's: the same operations in the same order.
+template <int NCH, int WPP>
+__global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, const double2* __restrict__ coeffs,
+                                                                 double2* __restrict__ G, double norm,
+                                                                 const double2* __restrict__ nq_grid,
+                                                                 const double2* __restrict__ nq_compact)
+{
+    if ((int)blockIdx.x >= t.npairs) {
+        nyquist_gather_one(nq_grid, G, t.B, nq_compact, (long)(blockIdx.x - t.npairs) * (64 * WPP) + threadIdx.x);
+        return;
+    }
+    extern __shared__ __align__(16) double sm[];
+    const int B = t.B, N = t.N;
+    const int2 pr = t.xpairs[blockIdx.x];
+    const int M = pr.x, Mp = pr.y;
+    const int R = B - M;
+    const int nch = (R + 15) >> 4;
+    const int RP = nch * 16;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int kw = w * 64 * NCH;
+    double* cf4 = sm;                            // RP x (f1, f1 f2, c1, 0)
+    double* stg = sm + 4 * RP + 256 * w;         // this wave's 16 degrees x 16 values
+
+    ChainTables<NCH> ct;
+    load_chain_tables<NCH>(ct, t, kw + lane, false);
+    const double nf = t.normfac[M * B + Mp];
+
+    // the right-hand side this lane broadcasts (j = lane % 16: case j/2,
+    // part j&1) and the staging role: lane L loads degrees L/16 + 4i of a
+    // chunk, whose parity is that of L/16
+    const int j = lane & 15, dl = lane >> 4;
+    const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
+    const double* csrc = reinterpret_cast<const double*>(coeffs) + (j & 1);
+    const double fac = lc.active ? norm * ((dl & 1) ? lc.sgn_odd : lc.sgn) : 0.0;
+    auto cidx = [&](int r) { return r < R ? 2 * coeff_index32(M + r, lc.Mc, lc.Mpc) : 0; };
+    double nxt[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) nxt[i] = csrc[cidx(dl + 4 * i)];
+
+    for (int r = tid; r < RP; r += 64 * WPP) {
+        recurrence_step(M, Mp, r, R, cf4[4 * r], cf4[4 * r + 1], cf4[4 * r + 2]);
+        cf4[4 * r + 3] = 0.0;
+    }
+    Chains<NCH> ch;
+    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+
+    double acc[NCH][16];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[c][q] = 0.0;
+    __syncthreads();  // the step-coefficient table is complete
+
+    // one degree: acc[c][q] += c_q * d(c) over the 16 right-hand sides, then
+    // the chains step to the next degree
+    auto degree = [&](double cv, double f1, double f1f2, double c1) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                if (q == 0 && c == 0) fmac_bcast<true>(q, acc[c][q], cv, ch.cur[c]);
+                else fmac_bcast<false>(q, acc[c][q], cv, ch.cur[c]);
+            }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const double u = fma(f1, ch.cb[c], f1f2);
+            const double dn = fma(ch.cur[c], u, c1 * ch.prev[c]);
+            ch.prev[c] = ch.cur[c];
+            ch.cur[c] = dn;
+        }
+    };
+    // stage a chunk's values (loaded a chunk ago); the wave's LDS operations
+    // run in order, so the writes land after the previous chunk's reads and
+    // before this chunk's
+    auto stage = [&]() {
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) stg[lane + 64 * i] = nxt[i] * fac;
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    // full chunks of 16 degrees, unrolled, each degree's LDS values read one
+    // degree ahead
+    const int nfull = R >> 4, rem = R & 15;
+    for (int chk = 0; chk < nfull; ++chk) {
+        stage();
+        const int rn = (chk + 1) * 16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) nxt[i] = csrc[cidx(rn + dl + 4 * i)];
+        const double* cfc = cf4 + 4 * 16 * chk;
+        double cvn = stg[j], c1n = cfc[2];
+        double2 s01n = *reinterpret_cast<const double2*>(cfc);
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const double cv = cvn, f1 = s01n.x, f1f2 = s01n.y, c1 = c1n;
+            if (rr < 15) {
+                cvn = stg[16 * (rr + 1) + j];
+                s01n = *reinterpret_cast<const double2*>(cfc + 4 * (rr + 1));
+                c1n = cfc[4 * (rr + 1) + 2];
+            }
+            degree(cv, f1, f1f2, c1);
+        }
+    }
+    // the last rem < 16 degrees one at a time (no padded degrees run)
+    if (rem) {
+        stage();
+        const double* cfc = cf4 + 4 * 16 * nfull;
+#pragma nounroll
+        for (int rr = 0; rr < rem; ++rr) {
+            const double2 s01 = *reinterpret_cast<const double2*>(cfc + 4 * rr);
+            degree(stg[16 * rr + j], s01.x, s01.y, cfc[4 * rr + 2]);
+        }
+    }
+
+    // each lane writes its nodes' values straight into the 8 pencils, as in
+    // k_dwt_inv_f64_valu (non-temporal only at 2B = 512)
+    const bool nts = N > 256;
+#pragma unroll
+    for (int cs = 0; cs < 8; ++cs) {
+        const CaseInfo co = case_info(cs, M, Mp, N);
+        if (!co.active) continue;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int k = kw + 64 * c + lane;
+            if (k < N) {
+                d2v val;
+                val.x = acc[c][2 * cs];
+                val.y = acc[c][2 * cs + 1];
+                d2v* dst = reinterpret_cast<d2v*>(G + co.vec + (co.flip ? N - 1 - k : k));
+                if (nts) __builtin_nontemporal_store(val, dst);
+                else *dst = val;
+            }
+        }
+    }
+}
+
 struct DwtShape {
     int nch, nh, wpp;
 };
@@ -962,6 +1118,7 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
     // its broadcasts folded into the FMAs
     const int kind = (variant >> 8) & 3;
     const bool dppfma = (variant >> 10) & 1;  // the broadcast folded into each FMA
+    const bool staged = (variant >> 11) & 1;  // k_dwt_inv_f64_staged (broadcasts folded as well)
     variant &= 0xff;
     if (kind) {
         // 2B = 512: four chains x two waves per pair, whatever the kind
@@ -972,6 +1129,21 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
         const long nq_total = (nq_grid || nq_compact) ? (long)(2 * t.N - 1) * t.N : 0;
         const int nq_blocks = (int)((nq_total + 64 * wpp - 1) / (64 * wpp));
         dim3 grid(t.npairs + nq_blocks), block(64 * wpp);
+        if (staged) {
+            if (qs != 1) return hipErrorInvalidValue;
+            const size_t ssmem = (4 * (size_t)(((t.B + 15) / 16) * 16) + 256 * (size_t)wpp) * sizeof(double);
+#define PFS_INVS(NC, W)                                                                                     \
+            if (nch == NC && wpp == W)                                                                      \
+                return pfs::launch((k_dwt_inv_f64_staged<NC, W>), grid, block, ssmem, st, t, coeffs, G, norm, nq_grid, \
+                                   nq_compact);
+            PFS_INVS(1, 1)
+            PFS_INVS(1, 2)
+            PFS_INVS(2, 2)
+            PFS_INVS(4, 1)
+            PFS_INVS(4, 2)
+#undef PFS_INVS
+            return hipErrorInvalidValue;
+        }
 #define PFS_INVV(NC, W, QS)                                                                                 \
         if (nch == NC && wpp == W && qs == QS) {                                                            \
             if (dppfma)                                                                                     \

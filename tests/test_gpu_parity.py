@@ -178,12 +178,13 @@ def test_dwt_work_shapes_match_oracle(ctx, b128, variant):
         ctx.tune("dwt_variant", 0)
 
 
-@pytest.mark.parametrize("kind", [0, 1, 2, 3, 5, 6, 7])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 5, 6, 7, 13, 15])
 def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
     # the MFMA inverse DWT (0) and the vector-ALU one in its three work shapes
     # (1: 2 chains x 2 waves over the nodes, 2: 4 chains x 2 waves over the
     # right-hand sides, 3: 4 x 1 per pair; 5-7 the same with the
-    # row broadcasts folded into the FMAs; grids of 2B <= 128
+    # row broadcasts folded into the FMAs; 13 and 15 the staged kernel in
+    # shapes 1 and 3; grids of 2B <= 128
     # use the shape that covers their nodes), at B = 128, at grids that leave
     # part of a shape's nodes empty (2B = 40, 80, 200), at B = 1..4, and through
     # the device route with stale Nyquist entries (copied by the same launch)
@@ -203,6 +204,40 @@ def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
         ctx.idsoft_dev(Bs, torch.from_numpy(cs).cuda(), gd, stream=torch.cuda.current_stream())
         torch.cuda.synchronize()
         assert rel(gd.cpu().numpy(), O.idsoft(Bs, cs, grid=stale)) < TOL
+    finally:
+        ctx.tune("dwt_inv_kernel", default)
+
+
+@pytest.mark.parametrize("pair", [(5, 13), (7, 15)])
+def test_staged_inverse_bit_identical(ctx, torch, pair):
+    # the staged inverse runs the vector-ALU kernel's operations in the same
+    # order, so the grids agree exactly (zeros up to their sign), at B = 128,
+    # 2B = 512 (four chains x two waves for both), sizes that leave chains or
+    # chunk rows empty, and through the device route with stale Nyquist entries
+    base, staged = pair
+    default = ctx.tune_get("dwt_inv_kernel")
+    try:
+        for B in (1, 3, 17, 40, 64, 100, 128, 256):
+            if B == 256 and base != 7:
+                continue
+            cs = O.seeded_coeffs(B, 500 + B)
+            out = []
+            for kind in (base, staged):
+                ctx.tune("dwt_inv_kernel", kind)
+                out.append(ctx.idsoft(B, cs))
+            assert np.array_equal(out[0], out[1]), B
+        Bs = 64
+        N = 2 * Bs
+        cs = torch.from_numpy(O.seeded_coeffs(Bs, 8)).cuda()
+        stale = np.random.default_rng(9).standard_normal(N ** 3) + 0j
+        grids = []
+        for kind in (base, staged):
+            ctx.tune("dwt_inv_kernel", kind)
+            gd = torch.from_numpy(stale.copy()).cuda()
+            ctx.idsoft_dev(Bs, cs, gd, stream=torch.cuda.current_stream())
+            torch.cuda.synchronize()
+            grids.append(gd.cpu().numpy())
+        assert np.array_equal(grids[0], grids[1])
     finally:
         ctx.tune("dwt_inv_kernel", default)
 
