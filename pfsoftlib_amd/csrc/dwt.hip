@@ -187,6 +187,51 @@ __device__ __forceinline__ double2 load_nt(const double2* p)
     return r;
 }
 
+// acc += (lane q of this lane's 16-lane row of c) * d, as one v_fmac_f64_dpp
+// (row_newbcast: DPP on 64-bit ALU ops; the compiler does not fold a separate
+// broadcast move into the FMA by itself).  The statements are volatile, so
+// they keep their program order: the first one after c is written (FIRST)
+// starts with the two wait states a DPP read of a VALU-written VGPR needs,
+// the same s_nop 1 the compiler places before its own DPP moves.
+#define PFS_FMAC_BCAST(Q)                                                                                   \
+    template <bool FIRST> __device__ __forceinline__ void fmac_bcast_##Q(double& acc, double c, double d)   \
+    {                                                                                                       \
+        if (FIRST)                                                                                          \
+            asm volatile("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:" #Q " row_mask:0xf bank_mask:0xf" \
+                         : "+v"(acc) : "v"(c), "v"(d));                                                     \
+        else                                                                                                \
+            asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #Q " row_mask:0xf bank_mask:0xf"           \
+                         : "+v"(acc) : "v"(c), "v"(d));                                                     \
+    }
+PFS_FMAC_BCAST(0) PFS_FMAC_BCAST(1) PFS_FMAC_BCAST(2) PFS_FMAC_BCAST(3)
+PFS_FMAC_BCAST(4) PFS_FMAC_BCAST(5) PFS_FMAC_BCAST(6) PFS_FMAC_BCAST(7)
+PFS_FMAC_BCAST(8) PFS_FMAC_BCAST(9) PFS_FMAC_BCAST(10) PFS_FMAC_BCAST(11)
+PFS_FMAC_BCAST(12) PFS_FMAC_BCAST(13) PFS_FMAC_BCAST(14) PFS_FMAC_BCAST(15)
+#undef PFS_FMAC_BCAST
+
+// q is a constant once the caller's loop is unrolled; the switch folds away.
+template <bool FIRST> __device__ __forceinline__ void fmac_bcast(int q, double& acc, double c, double d)
+{
+    switch (q) {
+    case 0: fmac_bcast_0<FIRST>(acc, c, d); break;
+    case 1: fmac_bcast_1<FIRST>(acc, c, d); break;
+    case 2: fmac_bcast_2<FIRST>(acc, c, d); break;
+    case 3: fmac_bcast_3<FIRST>(acc, c, d); break;
+    case 4: fmac_bcast_4<FIRST>(acc, c, d); break;
+    case 5: fmac_bcast_5<FIRST>(acc, c, d); break;
+    case 6: fmac_bcast_6<FIRST>(acc, c, d); break;
+    case 7: fmac_bcast_7<FIRST>(acc, c, d); break;
+    case 8: fmac_bcast_8<FIRST>(acc, c, d); break;
+    case 9: fmac_bcast_9<FIRST>(acc, c, d); break;
+    case 10: fmac_bcast_10<FIRST>(acc, c, d); break;
+    case 11: fmac_bcast_11<FIRST>(acc, c, d); break;
+    case 12: fmac_bcast_12<FIRST>(acc, c, d); break;
+    case 13: fmac_bcast_13<FIRST>(acc, c, d); break;
+    case 14: fmac_bcast_14<FIRST>(acc, c, d); break;
+    default: fmac_bcast_15<FIRST>(acc, c, d); break;
+    }
+}
+
 // Writes degrees r0 .. r0+15 of chains C0 .. C0+NC-1 into the tile (row rr,
 // column 64 (c - C0) + lane) and advances them.  Lane L fetches the step
 // coefficients of degree r0 + L % 16 in one LDS read each, so every 16-lane
@@ -209,6 +254,46 @@ __device__ __forceinline__ void run_chains(Chains<NCH>& ch, double* __restrict__
             ch.prev[c] = ch.cur[c];
             ch.cur[c] = dn;
         }
+    }
+}
+
+// run_chains with two of its three row broadcasts folded into DPP FMAs:
+// u = f1 f2 + f1 cb (the f1 f2 broadcast moved, f1 folded) and dn = cur u +
+// c1 prev (c1 folded), four VALU instructions per chain and degree where
+// run_chains takes six.  dn rounds cur u before adding c1 prev (run_chains
+// rounds c1 prev instead): the same accuracy, a different last bit.
+template <int NCH, int C0, int NC>
+__device__ __forceinline__ void run_chains_fold(Chains<NCH>& ch, double* __restrict__ tile, int tstr, int lane,
+                                                const double* __restrict__ cf, int RP, int r0)
+{
+    const int rl = r0 + (lane & 15);
+    const double vf1 = cf[rl], vf1f2 = cf[RP + rl], vc1 = cf[2 * RP + rl];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+        const double f1f2 = row_bcast(vf1f2, rr);
+#pragma unroll
+        for (int c = C0; c < C0 + NC; ++c) {
+            tile[rr * tstr + 64 * (c - C0) + lane] = ch.cur[c];
+            double u = f1f2;
+            // the DPP sources (vf1, vc1) come from LDS, not from a VALU write;
+            // the chunk's first fold waits the two states a DPP read needs anyway
+            if (rr == 0 && c == C0) fmac_bcast<true>(rr, u, vf1, ch.cb[c]);
+            else fmac_bcast<false>(rr, u, vf1, ch.cb[c]);
+            double dn = ch.cur[c] * u;
+            fmac_bcast<false>(rr, dn, vc1, ch.prev[c]);
+            ch.prev[c] = ch.cur[c];
+            ch.cur[c] = dn;
+        }
+    }
+}
+
+// Step coefficients of degrees r = first, first + stride, ... < RP,
+// interleaved per degree (cf4[4 r] = f1, f1 f2, c1, 0).
+__device__ __forceinline__ void recurrence_coeffs4(int M, int Mp, int R, int RP, double* cf4, int first, int stride)
+{
+    for (int r = first; r < RP; r += stride) {
+        recurrence_step(M, Mp, r, R, cf4[4 * r], cf4[4 * r + 1], cf4[4 * r + 2]);
+        cf4[4 * r + 3] = 0.0;
     }
 }
 
@@ -481,7 +566,7 @@ __device__ __forceinline__ void fwd_nb_pair(const PlanTables& t, const double2* 
         wave_sync();  // operand reads done before the recurrence writes the tile
 #pragma unroll
         for (int chk = 0; chk < NCHK; ++chk) {
-            run_chains<1, 0, 1>(ch, tile, TS, lane, cf, RP, chk * 16);
+            run_chains_fold<1, 0, 1>(ch, tile, TS, lane, cf, RP, chk * 16);
             wave_sync();
             v4d a1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -694,51 +779,6 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
-// acc += (lane q of this lane's 16-lane row of c) * d, as one v_fmac_f64_dpp
-// (row_newbcast: DPP on 64-bit ALU ops; the compiler does not fold a separate
-// broadcast move into the FMA by itself).  The statements are volatile, so
-// they keep their program order: the first one after c is written (FIRST)
-// starts with the two wait states a DPP read of a VALU-written VGPR needs,
-// the same s_nop 1 the compiler places before its own DPP moves.
-#define PFS_FMAC_BCAST(Q)                                                                                   \
-    template <bool FIRST> __device__ __forceinline__ void fmac_bcast_##Q(double& acc, double c, double d)   \
-    {                                                                                                       \
-        if (FIRST)                                                                                          \
-            asm volatile("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:" #Q " row_mask:0xf bank_mask:0xf" \
-                         : "+v"(acc) : "v"(c), "v"(d));                                                     \
-        else                                                                                                \
-            asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #Q " row_mask:0xf bank_mask:0xf"           \
-                         : "+v"(acc) : "v"(c), "v"(d));                                                     \
-    }
-PFS_FMAC_BCAST(0) PFS_FMAC_BCAST(1) PFS_FMAC_BCAST(2) PFS_FMAC_BCAST(3)
-PFS_FMAC_BCAST(4) PFS_FMAC_BCAST(5) PFS_FMAC_BCAST(6) PFS_FMAC_BCAST(7)
-PFS_FMAC_BCAST(8) PFS_FMAC_BCAST(9) PFS_FMAC_BCAST(10) PFS_FMAC_BCAST(11)
-PFS_FMAC_BCAST(12) PFS_FMAC_BCAST(13) PFS_FMAC_BCAST(14) PFS_FMAC_BCAST(15)
-#undef PFS_FMAC_BCAST
-
-// q is a constant once the caller's loop is unrolled; the switch folds away.
-template <bool FIRST> __device__ __forceinline__ void fmac_bcast(int q, double& acc, double c, double d)
-{
-    switch (q) {
-    case 0: fmac_bcast_0<FIRST>(acc, c, d); break;
-    case 1: fmac_bcast_1<FIRST>(acc, c, d); break;
-    case 2: fmac_bcast_2<FIRST>(acc, c, d); break;
-    case 3: fmac_bcast_3<FIRST>(acc, c, d); break;
-    case 4: fmac_bcast_4<FIRST>(acc, c, d); break;
-    case 5: fmac_bcast_5<FIRST>(acc, c, d); break;
-    case 6: fmac_bcast_6<FIRST>(acc, c, d); break;
-    case 7: fmac_bcast_7<FIRST>(acc, c, d); break;
-    case 8: fmac_bcast_8<FIRST>(acc, c, d); break;
-    case 9: fmac_bcast_9<FIRST>(acc, c, d); break;
-    case 10: fmac_bcast_10<FIRST>(acc, c, d); break;
-    case 11: fmac_bcast_11<FIRST>(acc, c, d); break;
-    case 12: fmac_bcast_12<FIRST>(acc, c, d); break;
-    case 13: fmac_bcast_13<FIRST>(acc, c, d); break;
-    case 14: fmac_bcast_14<FIRST>(acc, c, d); break;
-    default: fmac_bcast_15<FIRST>(acc, c, d); break;
-    }
-}
-
 // Inverse DWT on the vector ALU.  In the inverse, lane k owns node k's
 // outputs outright: for each degree l, out_j(k) += d_l(k) * c_l,j over the 16
 // right-hand sides j (8 cases x re/im), and c_l,j is the same for every lane.
@@ -930,10 +970,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
 #pragma unroll
     for (int i = 0; i < 4; ++i) nxt[i] = csrc[cidx(dl + 4 * i)];
 
-    for (int r = tid; r < RP; r += 64 * WPP) {
-        recurrence_step(M, Mp, r, R, cf4[4 * r], cf4[4 * r + 1], cf4[4 * r + 2]);
-        cf4[4 * r + 3] = 0.0;
-    }
+    recurrence_coeffs4(M, Mp, R, RP, cf4, tid, 64 * WPP);
     Chains<NCH> ch;
     init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
 
