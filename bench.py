@@ -47,6 +47,38 @@ def algorithmic_bytes_per_step(kernel, B, e=16):
     return 0
 
 
+def algorithmic_flops_per_step(kernel, B):
+    """Algorithmic FP64 flops of one transform in this kernel.  A DWT direction
+    contracts every one of the B(4B^2-1)/3 coefficients with its N-node
+    Wigner-d vector (complex x real: 4 flop per node), and builds those vectors
+    per transform as the reference does (fn_dsoft.cpp:227, fn_idsoft.cpp:224)
+    with the three-term recurrence: 5 flop per node, degree and canonical pair
+    (sum over pairs M >= M' of the B - M degrees).  The FFT passes' flops are
+    left out: at 5 log2 N flop per 16-byte point they are HBM-bound by two
+    orders of magnitude."""
+    N = 2 * B
+    if kernel.startswith("dwt_"):
+        rows = sum((M + 1) * (B - M) for M in range(B))
+        return 4 * N * (B * (4 * B * B - 1) // 3) + 5 * N * rows
+    return 0
+
+
+def roofline_of(alg_bytes, alg_flops, us):
+    """The binding roofline of one launch: the larger of its HBM floor
+    (algorithmic bytes at 8 TB/s) and its FP64 floor (algorithmic flops at
+    78.6 TFLOP/s, the matrix and the vector figure alike), with the achieved
+    rate and fraction against that peak, and both floors."""
+    t_hbm = alg_bytes / (HBM_PEAK_GBS * 1e9) * 1e6
+    t_fp64 = alg_flops / (FP64_PEAK_TFS * 1e12) * 1e6
+    hbm = {"achieved": round(alg_bytes / (us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(alg_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "floor_us": round(t_hbm, 2)}
+    if not alg_flops or t_fp64 <= t_hbm:
+        return "hbm", hbm, None
+    fp64 = {"achieved": round(alg_flops / (us * 1e-6) / 1e12, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+            "frac": round(alg_flops / (us * 1e-6) / 1e12 / FP64_PEAK_TFS, 4), "floor_us": round(t_fp64, 2)}
+    return "mfma", fp64, hbm
+
+
 def measured_traffic(B, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (profiles/pmc_traffic_f64_B<B>.json), or None."""
@@ -751,6 +783,26 @@ def main():
 
     lat = sorted(marks[i].elapsed_time(marks[i + 1]) for i in range(args.steps))
 
+    def roofline_line(name, k):
+        """The roofline object of the bench line for kernel `name` (timed-region
+        figures k): bound, achieved, peak, unit, frac against the binding floor
+        (FP64 for the DWT kernels, whose algorithmic flops need longer at
+        78.6 TFLOP/s than their bytes at 8 TB/s), the HBM figures beside it,
+        counter traffic and the counter-derived issue floor."""
+        nl = max(1, k["launches_per_step"])
+        ab = k["alg_bytes_per_step"] // nl
+        af = algorithmic_flops_per_step(name, B) // nl
+        bound, main, hbm = roofline_of(ab, af, k["avg_launch_us"])
+        line = {"bound": bound, "kernel": name, "achieved": main["achieved"], "peak": main["peak"],
+                "unit": main["unit"], "frac": main["frac"], "traffic": measured_traffic(B, name),
+                "alg_bytes": ab, "avg_launch_us": k["avg_launch_us"], "floor_us": main["floor_us"]}
+        if af:
+            line["alg_flops"] = af
+        if hbm:
+            line["hbm"] = hbm
+        line["issue"] = pipe_report(B, {name: k}).get(name)
+        return line
+
     def kernel_ms(names):
         return round(sum(kernels[k]["us_per_step"] for k in names if k in kernels) / 1e3, 4)
 
@@ -795,11 +847,7 @@ def main():
         "config": {"workload": "plain FP64 single-signal DSOFT+iDSOFT round trip, B=%d, %d^3 complex128 grid, "
                                "%d coefficients, inputs resident in HBM" % (B, N, P.coeff_count(B)),
                    "bandwidth": B, "grid": [N, N, N], "precision": "fp64", "signals_per_step": 1},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dk["frac_hbm"], "traffic": measured_traffic(B, dom),
-                     "alg_bytes": dk["alg_bytes_per_step"] // max(1, dk["launches_per_step"]),
-                     "avg_launch_us": dk["avg_launch_us"],
-                     "issue": pipe_report(B, {dom: dk}).get(dom)},
+        "roofline": roofline_line(dom, dk),
         "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))]},
         # kernel time per direction (sum of its kernels' average durations)
         "idsoft_ms": {"kernels_avg": kernel_ms(["dwt_inv", "nyquist_gather", "fft_inv_cols", "fft_inv_rows"])},
