@@ -334,13 +334,17 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0, prec=0):
         kernels[name] = k
     # FFT stage per direction, algorithmic: one read and one write of the grids
     # (2 N^3 e per signal) over the time of both passes of that direction
+    # (one launch of the one-pass slice kernel at 2B = 64 and 128, else two passes)
     fft_stage = {}
-    for d, names in (("forward", ("batch_fft_fwd_rows", "batch_fft_fwd_cols")),
-                     ("inverse", ("batch_fft_inv_cols", "batch_fft_inv_rows"))):
-        if all(x in kernels for x in names):
-            us = sum(kernels[x]["us_per_step"] for x in names)
-            gbs = nsig * 2 * N ** 3 * 8 / (us * 1e-6) / 1e9
-            fft_stage[d] = {"us": round(us, 2), "alg_GBps": round(gbs, 1), "alg_frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+    for d, alts in (("forward", (("batch_fft_fwd_slice",), ("batch_fft_fwd_rows", "batch_fft_fwd_cols"))),
+                    ("inverse", (("batch_fft_inv_slice",), ("batch_fft_inv_cols", "batch_fft_inv_rows")))):
+        for names in alts:
+            if all(x in kernels for x in names):
+                us = sum(kernels[x]["us_per_step"] for x in names)
+                gbs = nsig * 2 * N ** 3 * 8 / (us * 1e-6) / 1e9
+                fft_stage[d] = {"us": round(us, 2), "alg_GBps": round(gbs, 1),
+                                "alg_frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "kernels": list(names)}
+                break
 
     # error of every signal against the FP64 path on the same input
     g64 = torch.empty(N ** 3, dtype=torch.complex128, device="cuda")
@@ -692,6 +696,10 @@ def main():
     ap.add_argument("--batch-fft-signals", type=int, default=-1, help="batched FFT chunk: signals (-1 = library default)")
     ap.add_argument("--batch-fft-slices", type=int, default=-1, help="batched FFT chunk: beta-slices (-1 = library default)")
     ap.add_argument("--no-c4", action="store_true", help="skip the 512-signal B=64 batched leg")
+    ap.add_argument("--batch-fft-onepass", type=int, default=-1,
+                    help="batched FFT at 2B = 64, 128 in one pass (1) or two (0); -1 = library default")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched legs (for profiling them); prints no bench line")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="run the Poisson request stream through the request server for this long (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0, help="offered requests/s (0 = 70%% of measured capacity)")
@@ -728,6 +736,20 @@ def main():
         ctx.tune("batch_fft_signals", args.batch_fft_signals)
     if args.batch_fft_slices >= 0:
         ctx.tune("batch_fft_slices", args.batch_fft_slices)
+    if args.batch_fft_onepass >= 0:
+        ctx.tune("batch_fft_onepass", args.batch_fft_onepass)
+    if args.only_batched:
+        res = {}
+        if args.batch_signals and B <= 128:
+            res["batched_mixed"] = batched_leg(torch, P, ctx, B, args.batch_signals, args.batch_steps, 1, args.seed,
+                                               args.batch_check)
+            torch.cuda.empty_cache()
+        if not args.no_c4:
+            res["batched_mixed_512xB64"] = batched_leg(torch, P, ctx, 64, 512, args.batch_steps, 1, args.seed,
+                                                       args.batch_check)
+        print(json.dumps({"only_batched": res}))
+        ctx.close()
+        return
     power = args.power if args.power > 0 else (1.5 if B >= 256 else 1.0)
     coeffs = torch.from_numpy(P.seeded_coeffs(B, args.seed, power)).cuda()
     grid = torch.zeros(N ** 3, dtype=torch.complex128, device="cuda")

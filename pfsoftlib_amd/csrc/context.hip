@@ -296,6 +296,7 @@ struct pfs_context {
     long batch_force_shape = 0; // 0: tuned / default shapes; else 10 * waves + signals per wave
     long batch_fft_signals = 0; // batched FFT: signals per chunk (0 = the whole sub-batch)
     long batch_fft_slices = 0;  // batched FFT: beta-slices per chunk (0 = all); a multiple of 32
+    long batch_fft_onepass = 0; // batched FFT at 2B = 64, 128: one launch per direction, slices in LDS
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -1024,6 +1025,15 @@ int batch_fft_slices(const pfs_context* c, int N)
     return (k >= 32 && k < N && N % k == 0) ? (int)k : N;
 }
 
+// The one-pass slice FFT serves the batched complex64 passes at 2B = 64 and
+// 128 when the context turns it on.  It is off by default: measured on MI355X
+// (512 signals at 2B = 128) it takes 8.2 ms forward and 13.9 ms inverse against
+// 6.8 and 5.9 ms for the two passes.  Its pencil-side accesses are single
+// 8-byte values (one slice per workgroup, beta fastest in the pencils), one
+// L2 request per lane; with those accesses made contiguous (a timing probe
+// writing and reading the wrong layout) the same kernel takes 4.8 / 4.1 ms.
+bool use_slice_fft(const pfs_context* c, int N) { return c->batch_fft_onepass && fft_slice_supported(N, sizeof(float2)); }
+
 // Forward batch of n <= cap signals: FFT rows (grids -> W), FFT columns
 // (W -> G), DWT (G -> coefficients).  W and G hold cap grids each.
 int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSet coeffs, float2* W, float2* G,
@@ -1035,9 +1045,16 @@ int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSe
     // slice FFTs in chunks of S signals x K slices: the row pass of a chunk
     // writes a compact W block that the column pass reads back at once, so a
     // chunk small enough stays in the Infinity Cache instead of going through HBM
-    const int S = batch_fft_signals(c, n), K = batch_fft_slices(c, N);
+    const int S = use_slice_fft(c, N) ? n : batch_fft_signals(c, n), K = batch_fft_slices(c, N);
     const long n2 = (long)N * N;
-    for (int g0 = 0; g0 < n; g0 += S) {
+    if (use_slice_fft(c, N)) {
+        LineMap m{};
+        m.in_sig = grids.stride;
+        m.in_ptrs = (const void* const*)grids.ptrs;
+        m.out_sig = N3;
+        HIP_TRY(timed(c, "batch_fft_fwd_slice", st, [&] { return launch_fft_slice_f32(grids.base, G, m, N, -1, 1.0f, p->t.tw32, n, st); }));
+    }
+    for (int g0 = 0; g0 < n && !use_slice_fft(c, N); g0 += S) {
         const int ns = std::min(S, n - g0);
         const SigSet gs = grids.at(g0);
         for (int k0 = 0; k0 < N; k0 += K) {
@@ -1082,9 +1099,16 @@ int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, Sig
     } else {
         HIP_TRY(timed(c, "batch_dwt_inv", st, [&] { return launch_dwt_inv_batch_f32(p->t, p->bt, coeffs.base, G, n, coeffs.stride, 1, norm, shape_for(c, *p, false, n), st, (int)c->batch_precision); }));
     }
-    const int S = batch_fft_signals(c, n), K = batch_fft_slices(c, N);
+    const int S = use_slice_fft(c, N) ? n : batch_fft_signals(c, n), K = batch_fft_slices(c, N);
     const long n2 = (long)N * N;
-    for (int g0 = 0; g0 < n; g0 += S) {
+    if (use_slice_fft(c, N)) {
+        LineMap m{};
+        m.in_sig = N3;
+        m.out_sig = grids.stride;
+        m.out_ptrs = (void* const*)grids.ptrs;
+        HIP_TRY(timed(c, "batch_fft_inv_slice", st, [&] { return launch_fft_slice_f32(G, grids.base, m, N, +1, scale, p->t.tw32, n, st); }));
+    }
+    for (int g0 = 0; g0 < n && !use_slice_fft(c, N); g0 += S) {
         const int ns = std::min(S, n - g0);
         const SigSet gs = grids.at(g0);
         for (int k0 = 0; k0 < N; k0 += K) {
@@ -1179,7 +1203,19 @@ int multi_enqueue(pfs_context* c, bool fwd, const std::vector<MultiRun>& runs, c
     auto fft = [&](const MultiRun& r, bool forward) -> int {
         const int N = r.p->t.N, B = r.p->B;
         const long N3 = (long)N * N * N;
-        if (forward) {
+        if (use_slice_fft(c, N)) {
+            LineMap m{};
+            if (forward) {
+                m.in_ptrs = (const void* const*)r.in_ptrs;
+                m.out_sig = N3;
+                HIP_TRY(timed(c, "batch_fft_fwd_slice", st, [&] { return launch_fft_slice_f32(nullptr, r.G, m, N, -1, 1.0f, r.p->t.tw32, r.n, st); }));
+            } else {
+                m.in_sig = N3;
+                m.out_ptrs = (void* const*)r.out_ptrs;
+                const float scale = (float)(1. / (4. * B * B));
+                HIP_TRY(timed(c, "batch_fft_inv_slice", st, [&] { return launch_fft_slice_f32(r.G, nullptr, m, N, +1, scale, r.p->t.tw32, r.n, st); }));
+            }
+        } else if (forward) {
             LineMap a = map_rows_fwd(N), b = map_cols_fwd(N);
             a.in_ptrs = (const void* const*)r.in_ptrs;
             a.out_sig = b.in_sig = N3;
@@ -1802,6 +1838,16 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->batch_fft_slices = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "batch_fft_onepass")) {
+        if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "batch_fft_onepass must be 0 or 1");
+        if (value != c->batch_fft_onepass) {
+            // captured mixed batches hold the other kernels
+            (void)hipDeviceSynchronize();
+            drop_multi_graphs(c);
+        }
+        c->batch_fft_onepass = value;
+        return PFS_OK;
+    }
     if (!strcmp(key, "batch_force_shape")) {
         const BatchKernelShape sh{(int)value / 10, (int)value % 10};
         if (value != 0 && !(batch_shape_valid(256, true, sh) || batch_shape_valid(64, true, sh)))
@@ -1862,6 +1908,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_fft_slices")) {
         *value = c->batch_fft_slices;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_fft_onepass")) {
+        *value = c->batch_fft_onepass;
         return PFS_OK;
     }
     if (!strcmp(key, "batch_force_shape")) {

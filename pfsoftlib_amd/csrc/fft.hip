@@ -701,6 +701,160 @@ size_t lds_limit()
     return (size_t)cached[dev];
 }
 
+// ---------------------------------------------------------------------------
+// One-pass slice FFT: a workgroup holds one whole beta-slice in LDS and runs
+// both line passes of its 2D FFT there, so the grid crosses HBM once per
+// direction instead of twice (complex64, N = 64 and 128: a slice is 32 or
+// 128 KB).
+//
+// Forward, slice k of one signal:
+//   pass 1 (lines c, over r):  x[k][c][r]  (HBM, whole rows) -> Y[c][p]
+//   LDS transpose:             Y[c][p] -> Y'[p][c]
+//   pass 2 (lines p, over c):  Y'[p][c] -> G[p][q][k]  (HBM)
+// Inverse, slice k:
+//   pass 1 (lines p, over q):  G[p][q][k]  (HBM) -> Z[p][c]
+//   LDS transpose:             Z[p][c] -> Z'[c][p]
+//   pass 2 (lines c, over p):  Z'[c][p] -> x[k][c][r] * scale  (HBM, whole rows)
+// Each line runs the register kernel's four-step algorithm with the same
+// operations in the same order (16-point DFT, twiddle W_N^(n1 k2), exchange,
+// N/16-point DFTs), so the results equal the two-pass kernels' bit for bit.
+//
+// The pencil side moves one 8-byte value per (p, q) and slice: a 128-byte
+// line of G[p][q][k0..k0+15] is shared by 16 slices.  The workgroups are
+// therefore dealt so that the 16 slices of one such k-block run together on
+// one XCD (workgroup b runs on XCD b mod 8): their stores meet in that XCD's
+// L2 and leave it as whole lines, and their loads (inverse) fetch each line
+// once.  Both use ordinary (allocating) accesses for that reason.
+template <typename T, int LOGN> struct SliceGeo {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int TPL = N / 16;            // threads per line
+    static constexpr int THREADS = N * TPL;
+    static constexpr int YS = TPL + 1;            // exchange: stride between k2 rows
+    static constexpr int LS = 16 * YS + 1;        // exchange: stride between lines
+    static constexpr int RS = N + 8;              // transpose: row stride
+    static constexpr size_t LDS = (size_t)N * (LS > RS ? LS : RS) * sizeof(typename CT<T>::C);
+};
+
+// Workgroup b -> (signal, slice): XCD x = b mod 8 takes the k-blocks
+// g = 8 i + x, the 16 slices of block g at consecutive local positions.
+__device__ __forceinline__ bool slice_of_block(int b, long nblocks, int kblocks, long& sig, int& k)
+{
+    const int x = b & 7, j = b >> 3;
+    const long g = (long)(j >> 4) * 8 + x;
+    if (g >= nblocks) return false;
+    sig = g / kblocks;
+    k = (int)(g % kblocks) * 16 + (j & 15);
+    return true;
+}
+
+template <typename T, int LOGN, int SIGN>
+__global__ __launch_bounds__((SliceGeo<T, LOGN>::THREADS)) void k_fft_slice(const typename CT<T>::C* __restrict__ in,
+                                                                       typename CT<T>::C* __restrict__ out,
+                                                                       LineMap m,
+                                                                       const typename CT<T>::C* __restrict__ tw,
+                                                                       T scale, long nblocks)
+{
+    typedef typename CT<T>::C C;
+    typedef SliceGeo<T, LOGN> Geo;
+    constexpr int N = Geo::N, TPL = Geo::TPL, YS = Geo::YS, LS = Geo::LS, RS = Geo::RS;
+    constexpr int Q = 16 / TPL;
+    constexpr bool FWD = SIGN < 0;
+    extern __shared__ __align__(16) unsigned char smem_slice[];
+    C* S = reinterpret_cast<C*>(smem_slice);
+
+    long sig;
+    int k;
+    if (!slice_of_block(blockIdx.x, nblocks, N / 16, sig, k)) return;
+    const int tid = threadIdx.x;
+    const int line = tid / TPL, n = tid % TPL;  // line of either pass; n1 in step A, t in step B
+    // forward: x rows on the input side, pencils on the output side; inverse the reverse
+    const C* src = sig_in(in, m, sig) + (FWD ? (long)k * N * N : (long)k);
+    C* dst = sig_out(out, m, sig) + (FWD ? (long)k : (long)k * N * N);
+
+    C w[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) w[k2] = gload(tw, n * k2);
+    C v[16];
+    if (FWD) {
+        // whole rows x[k][line][n1 + TPL n2]
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = gload_nt(src, (unsigned)(line * N + n + TPL * n2));
+    } else {
+        // G[line][n1 + TPL n2][k]: one value per pencil, shared with the k-block's other slices
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = gload(src, (unsigned)((line * N + n + TPL * n2) * N));
+    }
+    C x[Q][TPL];
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            // step A input of the second pass: row `line` of the transposed slice
+#pragma unroll
+            for (int n2 = 0; n2 < 16; ++n2) v[n2] = S[line * RS + n + TPL * n2];
+            __syncthreads();  // every thread has read its row before the exchange overwrites it
+        }
+        dft16_perm<SIGN>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) S[line * LS + k2 * YS + n] = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int a = 0; a < TPL; ++a) x[q][a] = S[line * LS + (n + TPL * q) * YS + a];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) dft_small<TPL, SIGN>(x[q]);
+        if (pass == 0) {
+            __syncthreads();  // exchange reads done before the transpose writes
+            // output (line, f = k2 + 16 k1) of pass 1 goes to row f, column `line`
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+#pragma unroll
+                for (int k1 = 0; k1 < TPL; ++k1) S[(n + TPL * q + 16 * k1) * RS + line] = x[q][perm_index<TPL>(k1)];
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int k1 = 0; k1 < TPL; ++k1) {
+            const int f = n + TPL * q + 16 * k1;
+            C r = x[q][perm_index<TPL>(k1)];
+            r.x *= scale;
+            r.y *= scale;
+            if (FWD) gstore(dst, (unsigned)((line * N + f) * N), r);  // G[line][f][k]
+            else gstore(dst, (unsigned)(line * N + f), r);            // x[k][line][f]
+        }
+}
+
+bool fft_slice_supported(int N, size_t elem_bytes)
+{
+    return elem_bytes == sizeof(float2) && (N == 64 || N == 128) &&
+           (N == 64 ? SliceGeo<float, 6>::LDS : SliceGeo<float, 7>::LDS) <= lds_limit();
+}
+
+template <int LOGN>
+static hipError_t launch_slice_f32(const float2* in, float2* out, const LineMap& m, int sign, float scale,
+                                   const float2* tw, int nsig, hipStream_t st)
+{
+    typedef SliceGeo<float, LOGN> Geo;
+    const long nblocks = (long)nsig * (Geo::N / 16);
+    const long nwg = (nblocks + 7) / 8 * 8 * 16;
+    if (nwg > 0x7fffffffL) return hipErrorInvalidValue;
+    if (sign < 0)
+        return pfs::launch((k_fft_slice<float, LOGN, -1>), dim3((unsigned)nwg), dim3(Geo::THREADS), Geo::LDS, st, in, out,
+                           m, tw, scale, nblocks);
+    return pfs::launch((k_fft_slice<float, LOGN, 1>), dim3((unsigned)nwg), dim3(Geo::THREADS), Geo::LDS, st, in, out, m,
+                       tw, scale, nblocks);
+}
+
+hipError_t launch_fft_slice_f32(const float2* in, float2* out, const LineMap& m, int N, int sign, float scale,
+                                const float2* tw, int nsig, hipStream_t st)
+{
+    if (nsig < 1 || !fft_slice_supported(N, sizeof(float2))) return hipErrorInvalidValue;
+    return N == 64 ? launch_slice_f32<6>(in, out, m, sign, scale, tw, nsig, st)
+                   : launch_slice_f32<7>(in, out, m, sign, scale, tw, nsig, st);
+}
+
 hipError_t launch_fft_lines_f64(const double2* in, double2* out, const LineMap& m, int N, int sign, double scale,
                                 const double2* tw, int kind, int nsig, hipStream_t st)
 {

@@ -164,6 +164,12 @@ hipError_t launch_fft_lines_f64(const double2* in, double2* out, const LineMap& 
                                 double scale, const double2* tw, int kind, int nsig, hipStream_t st);
 hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m, int N, int sign,
                                 float scale, const float2* tw, int kind, int nsig, hipStream_t st);
+// One-pass slice FFT (complex64, N = 64 or 128): both line passes of every
+// slice's 2D FFT in one launch.  sign -1: grids (input side of m) -> pencils
+// G[p][q][k] (output side); sign +1: pencils -> grids, times scale.
+bool fft_slice_supported(int N, size_t elem_bytes);
+hipError_t launch_fft_slice_f32(const float2* in, float2* out, const LineMap& m, int N, int sign, float scale,
+                                const float2* tw, int nsig, hipStream_t st);
 
 // ---- DWT stage -----------------------------------------------------------
 // Forward: G pencils (FP64) -> packed coefficients, scaled by pi/(2B^2).

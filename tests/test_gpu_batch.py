@@ -147,6 +147,7 @@ def test_chunked_batch_fft_is_bitwise_invariant(ctx, torch, B, nsig):
     cs = np.concatenate([O.seeded_coeffs(B, 80 + i) for i in range(nsig)]).astype(np.complex64)
     cdev = torch.from_numpy(cs).cuda()
     outs = []
+    ctx.tune("batch_fft_onepass", 0)  # the chunk knobs steer the two-pass kernels (the default)
     for S, K in [(0, 0), (2, 32), (3, 64), (1, 0), (0, 64)]:
         ctx.tune("batch_fft_signals", S)
         ctx.tune("batch_fft_slices", K)
@@ -160,6 +161,37 @@ def test_chunked_batch_fft_is_bitwise_invariant(ctx, torch, B, nsig):
     torch.cuda.synchronize()
     for g, c in outs[1:]:
         assert torch.equal(g, outs[0][0]) and torch.equal(c, outs[0][1])
+
+
+@pytest.mark.parametrize("B,nsig", [(32, 3), (32, 8), (64, 5), (64, 17)])
+def test_onepass_slice_fft_matches_two_pass(ctx, torch, B, nsig):
+    """The one-pass slice FFT (both line passes of a slice's 2D FFT inside one
+    workgroup) runs every line with the two-pass kernels' operations in the
+    same order, so grids and coefficients equal the two-pass results bit for
+    bit in both directions.  Signal counts whose k-blocks do not fill the
+    last group of eight XCD slots (B = 32 x 3, B = 64 x 17) exercise the
+    workgroups that find no slice."""
+    N = 2 * B
+    s = torch.cuda.current_stream()
+    cs = np.concatenate([O.seeded_coeffs(B, 300 + i) for i in range(nsig)]).astype(np.complex64)
+    cdev = torch.from_numpy(cs).cuda()
+    # a seeded non-band-limited grid batch for the forward direction
+    rng = np.random.default_rng(7 + nsig)
+    gin = (rng.uniform(-1, 1, nsig * N ** 3) + 1j * rng.uniform(-1, 1, nsig * N ** 3)).astype(np.complex64)
+    gdev = torch.from_numpy(gin).cuda()
+    outs = {}
+    for one in (0, 1):
+        ctx.tune("batch_fft_onepass", one)
+        g = torch.full((nsig * N ** 3,), float("nan"), dtype=torch.complex64, device="cuda")
+        ctx.idsoft_batch_f32(B, nsig, cdev, g, stream=s)
+        c = torch.full_like(cdev, float("nan"))
+        ctx.dsoft_batch_f32(B, nsig, gdev, c, stream=s)
+        outs[one] = (g, c)
+    ctx.tune("batch_fft_onepass", 0)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
+    assert not torch.isnan(outs[1][0]).any() and not torch.isnan(outs[1][1]).any()
 
 
 @pytest.mark.parametrize("prec,bound", [(6, 1e-6), (3, 2e-5)])
