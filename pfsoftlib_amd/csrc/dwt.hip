@@ -1,8 +1,9 @@
 // Discrete Wigner transform stage, FP64, Wigner-d generated on the fly.
 //
-// One wave per canonical pair (M, M'), M >= M' >= 0, for 2B <= 256 nodes
-// (dwt_large.hip takes the larger forward grids; the vector-ALU inverse
-// below, k_dwt_inv_f64_valu, the default inverse, also runs 2B = 512).  Lane i of the wave runs the three-term recurrence of the
+// One wave per canonical pair (M, M'), M >= M' >= 0 (the default kernels: the
+// node-blocked forward k_dwt_fwd_f64_nb and the staged vector-ALU inverse
+// k_dwt_inv_f64_staged, at every 2B up to 512; the chain-parallel kernels up
+// to 2B = 256).  Lane i of the wave runs the three-term recurrence of the
 // L2-normalised Wigner d-function (fn_dwt.hpp:152-199: base row, then rows
 // l = M+1 .. B-1) in FP64 registers for the beta nodes k = i, i+64, ... --
 // one recurrence chain per 64 nodes.  Sixteen degrees of all chains at a time
@@ -1068,7 +1069,7 @@ struct DwtShape {
     int nch, nh, wpp;
 };
 
-// Grids up to 256 nodes (larger ones go to dwt_large.hip).  Default forward
+// Chain-parallel kernels: grids up to 256 nodes.  Default forward
 // shape: one wave per pair, chains two at a time through a 16 KB tile so two
 // waves fit per SIMD; inverse: two chains per lane and one wave per 128 nodes
 // (its waves need no reduction).  Other variants (tuning knob "dwt_variant",
@@ -1103,8 +1104,14 @@ static size_t dwt_smem(int B, int N, bool fwd, int variant)
 hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* coeffs, double norm, hipStream_t st,
                               int variant)
 {
-    if (t.N > 1024) return hipErrorInvalidValue;
-    if (t.N > 256) return launch_dwt_fwd_f64_large(t, G, coeffs, norm, st);
+    if (t.N > 512) return hipErrorInvalidValue;
+    if (t.N == 512) {
+        // every variant: the node-blocked kernel, 8 blocks of 64 nodes and up
+        // to 16 chunk accumulators per lane (248 VGPRs, two waves per SIMD).
+        // Measured at B = 256: 1.25 ms against 1.75 ms for the chain-parallel
+        // four-chains-by-two-waves kernel that served 2B = 512 before.
+        return pfs::launch((k_dwt_fwd_f64_nb<8, 2>), dim3(t.npairs), dim3(64), fwd_nb_smem(t.B), st, t, G, coeffs, norm);
+    }
     // Default at 2B = 128 and 256: the node-blocked kernel, held to three waves
     // per SIMD at 2B = 256 (168 VGPRs, 52 bytes of spills; measured 101.5 us
     // against 105.0 us for the chain-parallel kernel and 109.5 us at two waves)
@@ -1143,10 +1150,11 @@ hipError_t launch_dwt_fwd_f64(const PlanTables& t, const double2* G, double2* co
 hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double2* G, double norm, hipStream_t st,
                               int variant, const double2* nq_grid, const double2* nq_compact)
 {
-    if (t.N > 1024) return hipErrorInvalidValue;
+    if (t.N > 512) return hipErrorInvalidValue;
     if (t.N > 256) {
         if (nq_grid || nq_compact) return hipErrorInvalidValue;  // gathered by its own kernel there
-        if (!((variant >> 8) & 3) || t.N > 512) return launch_dwt_inv_f64_large(t, coeffs, G, norm, st);
+        // the MFMA inverse serves 2B <= 256; the vector-ALU kernels take 2B = 512
+        if (!((variant >> 8) & 3)) variant = (variant & 0xff) | (15 << 8);
     }
     // bits 8-9 of variant: 0 = the MFMA kernel (shape from the low bits),
     // 1..3 = the vector-ALU kernel with 2 chains x 2 waves over the nodes,

@@ -184,11 +184,6 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
                               hipStream_t st, int variant = 0, const double2* nq_grid = nullptr,
                               const double2* nq_compact = nullptr);
 
-// Variants for 2B >= 512 (dwt_large.hip).
-hipError_t launch_dwt_fwd_f64_large(const PlanTables& t, const double2* G, double2* coeffs, double norm,
-                                    hipStream_t st);
-hipError_t launch_dwt_inv_f64_large(const PlanTables& t, const double2* coeffs, double2* G, double norm,
-                                    hipStream_t st);
 
 // ---- batched mixed-precision path (batch.hip) ------------------------------
 bool batch_supported(int N);
