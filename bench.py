@@ -101,6 +101,20 @@ def measured_pipes(B):
         return {}
 
 
+def batched_counters(B, nsig):
+    """Committed rocprofv3 counters of the batched kernels at this workload
+    (profiles/pmc_batched.json: 64 signals at B=128, 512 at B=64), or {}."""
+    key = {(128, 64): "64xB128", (64, 512): "512xB64"}.get((B, nsig))
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_batched.json")
+    if not key:
+        return {}
+    try:
+        with open(path) as f:
+            return json.load(f)["workloads"][key]["kernels"]
+    except (OSError, KeyError, ValueError):
+        return {}
+
+
 def pipe_report(B, kernels):
     """For each kernel with committed SQ counters: the SIMD issue floor (VALU
     and MFMA cycles the counters give, spread over every SIMD at the peak
@@ -334,6 +348,39 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0, prec=0):
         kernels[name] = k
     # FFT stage per direction, algorithmic: one read and one write of the grids
     # (2 N^3 e per signal) over the time of both passes of that direction
+    # committed counters of these kernels (traffic per launch from FETCH_SIZE /
+    # WRITE_SIZE, MFMA busy cycles), set against this run's launch times: the
+    # MFMA pipes' busy fraction of every SIMD at the 2.4 GHz peak clock over
+    # the live launch, and the leg's counter traffic against its algorithmic
+    # bytes per round trip
+    cnt = batched_counters(B, nsig)
+    counter_summary = None
+    if cnt:
+        traffic_step = 0
+        for name, k in kernels.items():
+            key = name
+            if name.startswith("batch_fft_fwd"):
+                key = "batch_fft_fwd_rows+cols"
+            elif name.startswith("batch_fft_inv"):
+                key = "batch_fft_inv_cols+rows"
+            c = cnt.get(key)
+            if name == "batch_coeff_transpose":
+                # one read + one write of the coefficient blocks per launch (counters: 1.000x)
+                traffic_step += 2 * 2 * nsig * (B * (4 * B * B - 1) // 3) * 8
+            if not c:
+                continue
+            k["counter_traffic_bytes_per_launch"] = int(c["traffic_bytes"])
+            k["counter_traffic_over_alg"] = c["traffic_over_alg"]
+            traffic_step += c["traffic_bytes"] * k["launches_per_step"]
+            if "mfma_busy_cycles" in c:
+                k["mfma_busy_counter_run"] = c["mfma_busy_run"]
+                k["mfma_busy_live"] = round(c["mfma_busy_cycles"] / (1024 * 2.4e9 * k["avg_launch_us"] * 1e-6), 4)
+        alg_step = nsig * 2 * (2 * N ** 3 * 8 + ((N - 1) ** 2 * N + B * (4 * B * B - 1) // 3) * 8)
+        counter_summary = {"traffic_bytes_per_roundtrip": int(traffic_step), "alg_bytes_per_roundtrip": alg_step,
+                           "traffic_over_alg": round(traffic_step / alg_step, 3),
+                           "note": "counter figures from profiles/pmc_batched.json (rocprofv3 2 x FETCH_SIZE + "
+                                   "WRITE_SIZE per launch, SQ_VALU_MFMA_BUSY_CYCLES); mfma_busy_live = committed "
+                                   "busy cycles / (1024 SIMDs x 2.4 GHz x this run's launch time)"}
     # (one launch of the one-pass slice kernel at 2B = 64 and 128, else two passes)
     fft_stage = {}
     for d, alts in (("forward", (("batch_fft_fwd_slice",), ("batch_fft_fwd_rows", "batch_fft_fwd_cols"))),
@@ -374,6 +421,7 @@ def batched_leg(torch, P, ctx, B, nsig, steps, warmup, seed, check=0, prec=0):
                            "signals_checked": nchk, "bound": 1e-6},
         "kernels": kernels,
         "fft_stage": fft_stage,
+        "counters": counter_summary,
     }
 
 
@@ -652,19 +700,22 @@ def dwt_accuracy(ctx, np, max_bw, runs, seed):
 
 
 def copy_calibration(torch, P, steps, mib):
-    """Copies of known size for scaling PMC byte counters: the library's
-    16-byte copy kernel (the FFT passes' access width) with ordinary and with
-    non-temporal loads and stores (kernels k_copy16<false> / <true>)."""
+    """Copies of known size for scaling PMC byte counters: the library's copy
+    kernels with the FFT passes' access widths (16 bytes for complex128, 8 for
+    complex64), each with ordinary and with non-temporal loads and stores
+    (kernels k_copy16<false> / <true>, k_copy8<false> / <true>)."""
     ctx = P.Context(0)
     n = mib * (1 << 20)
     a = torch.ones(n // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
-    for nt in (0, 1):
-        for _ in range(steps):
-            ctx.copy_dev(b, a, n, nontemporal=nt)
+    for width in (16, 8):
+        for nt in (0, 1):
+            for _ in range(steps):
+                ctx.copy_dev(b, a, n, nontemporal=nt, width=width)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches_per_kernel": steps,
-                                           "kernels": ["k_copy16<false>", "k_copy16<true>"]}}))
+                                           "kernels": ["k_copy16<false>", "k_copy16<true>", "k_copy8<false>",
+                                                       "k_copy8<true>"]}}))
     ctx.close()
 
 

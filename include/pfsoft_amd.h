@@ -272,11 +272,13 @@ int pfs_device_alloc(pfs_context* ctx, size_t bytes, void** out);
 int pfs_device_free(pfs_context* ctx, void* ptr);
 int pfs_memcpy_h2d(pfs_context* ctx, void* dst_dev, const void* src_host, size_t bytes);
 int pfs_memcpy_d2h(pfs_context* ctx, void* dst_host, const void* src_dev, size_t bytes);
-/* dst <- src on stream, bytes a multiple of 16, in 16-byte accesses: the
- * access width and (nontemporal != 0) the cache hints of the FFT passes.  A
- * copy of known size for calibrating hardware byte counters against the
- * kernels' own access pattern (no counterpart in the reference). */
-int pfs_copy_dev(pfs_context* ctx, void* dst_dev, const void* src_dev, size_t bytes, int nontemporal, void* stream);
+/* dst <- src on stream, bytes a multiple of 16, in 16-byte accesses (the
+ * access width of the complex128 FFT passes) or, with flags bit 1 (2), 8-byte
+ * accesses (the complex64 passes); flags bit 0 (1): non-temporal loads and
+ * stores, the cache hints of those passes.  A copy of known size for
+ * calibrating hardware byte counters against the kernels' own access pattern
+ * (no counterpart in the reference). */
+int pfs_copy_dev(pfs_context* ctx, void* dst_dev, const void* src_dev, size_t bytes, int flags, void* stream);
 /* Waits for stream; NULL waits for the context's own stream and the null stream. */
 int pfs_synchronize(pfs_context* ctx, void* stream);
 

@@ -353,10 +353,12 @@ class Context:
         ptr = _devbuf(arr_dev, "arr", "c128", max(int(n), 0) ** 2 * max(int(lays), 0), self.device)
         _check(lib().pfs_layer_wise_dft2_dev(self._h, n, lays, ptr, sign, scale, _stream_handle(stream)))
 
-    def copy_dev(self, dst, src, nbytes, nontemporal=False, stream=None):
-        """dst <- src (device addresses or tensors), 16-byte accesses, optionally non-temporal."""
-        _check(lib().pfs_copy_dev(self._h, _devptr(dst), _devptr(src), nbytes, 1 if nontemporal else 0,
-                                  _stream_handle(stream)))
+    def copy_dev(self, dst, src, nbytes, nontemporal=False, stream=None, width=16):
+        """dst <- src (device addresses or tensors), 16- or 8-byte accesses, optionally non-temporal."""
+        if width not in (8, 16):
+            raise ValueError("copy_dev: width must be 8 or 16 bytes")
+        flags = (1 if nontemporal else 0) | (2 if width == 8 else 0)
+        _check(lib().pfs_copy_dev(self._h, _devptr(dst), _devptr(src), nbytes, flags, _stream_handle(stream)))
 
     def synchronize(self, stream=None):
         _check(lib().pfs_synchronize(self._h, _stream_handle(stream)))

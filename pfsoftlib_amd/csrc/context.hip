@@ -1439,6 +1439,20 @@ __global__ void k_copy16(const double2* __restrict__ src, double2* __restrict__ 
     }
 }
 
+// The same with 8-byte accesses (one complex64 value per lane, the width of
+// the batched FP32 FFT passes).
+template <bool NT>
+__global__ void k_copy8(const float2* __restrict__ src, float2* __restrict__ dst, long n)
+{
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    const v2f* s = reinterpret_cast<const v2f*>(src);
+    v2f* d = reinterpret_cast<v2f*>(dst);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
+        else d[i] = s[i];
+    }
+}
+
 extern "C" {
 
 const char* pfs_last_error(void) { return g_last_error.c_str(); }
@@ -2016,12 +2030,23 @@ int pfs_memcpy_d2h(pfs_context* c, void* dst, const void* src, size_t bytes)
     return PFS_OK;
 }
 
-int pfs_copy_dev(pfs_context* c, void* dst_dev, const void* src_dev, size_t bytes, int nontemporal, void* stream)
+int pfs_copy_dev(pfs_context* c, void* dst_dev, const void* src_dev, size_t bytes, int flags, void* stream)
 {
     if (!c || !dst_dev || !src_dev || (bytes & 15) || ((uintptr_t)dst_dev & 15) || ((uintptr_t)src_dev & 15))
         return fail(PFS_ERR_ARG, "pfs_copy_dev: null argument, or a size or address not a multiple of 16 bytes");
+    if (flags & ~3) return fail(PFS_ERR_ARG, "pfs_copy_dev: flags other than 1 (non-temporal) and 2 (8-byte accesses)");
     std::lock_guard<std::mutex> g(c->mu);
     HIP_TRY(hipSetDevice(c->device));
+    const bool nontemporal = flags & 1;
+    if (flags & 2) {
+        const long n8 = (long)(bytes / 8);
+        const int blocks8 = (int)std::min<long>((n8 + 255) / 256, 8192);
+        if (nontemporal)
+            HIP_TRY(timed(c, "copy8_nt", pick(c, stream), [&] { return pfs::launch(k_copy8<true>, dim3(blocks8), dim3(256), 0, pick(c, stream), (const float2*)src_dev, (float2*)dst_dev, n8); }));
+        else
+            HIP_TRY(timed(c, "copy8", pick(c, stream), [&] { return pfs::launch(k_copy8<false>, dim3(blocks8), dim3(256), 0, pick(c, stream), (const float2*)src_dev, (float2*)dst_dev, n8); }));
+        return PFS_OK;
+    }
     const long n = (long)(bytes / 16);
     const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
     if (n == 0) return PFS_OK;
