@@ -749,6 +749,9 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the 512-signal B=64 batched leg")
     ap.add_argument("--batch-fft-onepass", type=int, default=-1,
                     help="batched FFT at 2B = 64, 128 in one pass (1) or two (0); -1 = library default")
+    ap.add_argument("--batch-fft-wide", type=int, default=-1,
+                    help="batched FFT passes with 16-byte accesses: 0 none, 1 all, 2 forward passes of 2B <= 128; "
+                         "-1 = library default")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched legs (for profiling them); prints no bench line")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
@@ -789,6 +792,8 @@ def main():
         ctx.tune("batch_fft_slices", args.batch_fft_slices)
     if args.batch_fft_onepass >= 0:
         ctx.tune("batch_fft_onepass", args.batch_fft_onepass)
+    if args.batch_fft_wide >= 0:
+        ctx.tune("batch_fft_wide", args.batch_fft_wide)
     if args.only_batched:
         res = {}
         if args.batch_signals and B <= 128:

@@ -212,7 +212,8 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
 /* Tuning knobs (no counterpart in the reference).  Keys:
  *   "batch_interleave"  1 (default): batched contraction on signal-interleaved
  *                       coefficients with transposes; 0: on the caller's layout.
- *   "dwt_variant"       work shape of the FP64 DWT kernels for 2B <= 256
+ *   "dwt_variant"       work shape of the FP64 DWT kernels for 2B <= 256 (2B = 512
+ *                       always runs the node-blocked forward)
  *                       (0 default, 1..6 alternatives; results agree to
  *                       rounding).  Forward at 2B = 128, 256: 0 the
  *                       node-blocked kernel (one 64-node block at a time,
@@ -240,6 +241,12 @@ int pfs_stage_fft_inv_dev(pfs_context* ctx, int bandwidth, const void* pencils_d
  *                       Cache at the price of more, smaller launches.
  *   "batch_chunk_signals" signals per sub-batch of the batched path (work
  *                       buffers hold this many complex64 grids; default 64).
+ *   "batch_fft_wide"    complex64 FFT passes with 16-byte accesses (two values
+ *                       per thread): 2 (default) the forward passes of 2B <= 128,
+ *                       1 every pass, 0 none; results equal bit for bit.
+ *   "batch_fft_onepass" 1: the batched complex64 FFT at 2B = 64 and 128 as one
+ *                       launch per direction (whole slices in LDS); 0 (default)
+ *                       two passes; results equal bit for bit.
  *   "batch_graphs"      mixed-bandwidth batches replay a captured HIP graph
  *                       per composition (1, default) or launch directly (0).
  *   "batch_precision"   products of the batched Wigner contraction: 0 FP32

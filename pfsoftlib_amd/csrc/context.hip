@@ -297,6 +297,7 @@ struct pfs_context {
     long batch_fft_signals = 0; // batched FFT: signals per chunk (0 = the whole sub-batch)
     long batch_fft_slices = 0;  // batched FFT: beta-slices per chunk (0 = all); a multiple of 32
     long batch_fft_onepass = 0; // batched FFT at 2B = 64, 128: one launch per direction, slices in LDS
+    long batch_fft_wide = 2;    // batched FFT passes with 16-byte accesses: 0 none, 1 all, 2 forward passes of 2B <= 128
     // batched-path work buffers (complex64, batch_chunk_signals grids each)
     void* bwork[2] = {nullptr, nullptr};
     size_t bwork_bytes = 0;
@@ -1067,8 +1068,8 @@ int dsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet grids, SigSe
             b.nv = K;
             b.out_sig = N3;
             float2* Gc = G + g0 * N3 + k0;
-            HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(gs.base, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, ns, st); }));
-            HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, Gc, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, ns, st); }));
+            HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(gs.base, W, a, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, ns, st, (int)c->batch_fft_wide); }));
+            HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(W, Gc, b, N, -1, 1.0f, p->t.tw32, LINES_ROW_IN_TILE_OUT, ns, st, (int)c->batch_fft_wide); }));
         }
     }
     if (c->batch_interleave || coeffs.ptrs) {
@@ -1121,8 +1122,8 @@ int idsoft_batch_run(pfs_context* c, Plan* p, int n, int cap, SigSet coeffs, Sig
             a.out_ptrs = (void* const*)gs.ptrs;
             a.out_off = k0 * n2;
             const float2* Gc = G + g0 * N3 + k0;
-            HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(Gc, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, ns, st); }));
-            HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, gs.base, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, ns, st); }));
+            HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(Gc, W, b, N, +1, 1.0f, p->t.tw32, LINES_TILE_IN_ROW_OUT, ns, st, (int)c->batch_fft_wide); }));
+            HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(W, gs.base, a, N, +1, scale, p->t.tw32, LINES_TILE_IN_ROW_OUT, ns, st, (int)c->batch_fft_wide); }));
         }
     }
     return PFS_OK;
@@ -1220,16 +1221,16 @@ int multi_enqueue(pfs_context* c, bool fwd, const std::vector<MultiRun>& runs, c
             a.in_ptrs = (const void* const*)r.in_ptrs;
             a.out_sig = b.in_sig = N3;
             b.out_sig = N3;
-            HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(nullptr, r.W, a, N, -1, 1.0f, r.p->t.tw32, LINES_ROW_IN_TILE_OUT, r.n, st); }));
-            HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(r.W, r.G, b, N, -1, 1.0f, r.p->t.tw32, LINES_ROW_IN_TILE_OUT, r.n, st); }));
+            HIP_TRY(timed(c, "batch_fft_fwd_rows", st, [&] { return launch_fft_lines_f32(nullptr, r.W, a, N, -1, 1.0f, r.p->t.tw32, LINES_ROW_IN_TILE_OUT, r.n, st, (int)c->batch_fft_wide); }));
+            HIP_TRY(timed(c, "batch_fft_fwd_cols", st, [&] { return launch_fft_lines_f32(r.W, r.G, b, N, -1, 1.0f, r.p->t.tw32, LINES_ROW_IN_TILE_OUT, r.n, st, (int)c->batch_fft_wide); }));
         } else {
             LineMap b = map_cols_inv(N), a = map_rows_inv(N);
             b.in_sig = N3;
             b.out_sig = a.in_sig = N3;
             a.out_ptrs = (void* const*)r.out_ptrs;
             const float scale = (float)(1. / (4. * B * B));
-            HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(r.G, r.W, b, N, +1, 1.0f, r.p->t.tw32, LINES_TILE_IN_ROW_OUT, r.n, st); }));
-            HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(r.W, nullptr, a, N, +1, scale, r.p->t.tw32, LINES_TILE_IN_ROW_OUT, r.n, st); }));
+            HIP_TRY(timed(c, "batch_fft_inv_cols", st, [&] { return launch_fft_lines_f32(r.G, r.W, b, N, +1, 1.0f, r.p->t.tw32, LINES_TILE_IN_ROW_OUT, r.n, st, (int)c->batch_fft_wide); }));
+            HIP_TRY(timed(c, "batch_fft_inv_rows", st, [&] { return launch_fft_lines_f32(r.W, nullptr, a, N, +1, scale, r.p->t.tw32, LINES_TILE_IN_ROW_OUT, r.n, st, (int)c->batch_fft_wide); }));
         }
         return PFS_OK;
     };
@@ -1852,6 +1853,15 @@ int pfs_tune_set(pfs_context* c, const char* key, long value)
         c->batch_fft_slices = value;
         return PFS_OK;
     }
+    if (!strcmp(key, "batch_fft_wide")) {
+        if (value < 0 || value > 2) return fail(PFS_ERR_ARG, "batch_fft_wide must be 0, 1 or 2");
+        if (value != c->batch_fft_wide) {
+            (void)hipDeviceSynchronize();
+            drop_multi_graphs(c);
+        }
+        c->batch_fft_wide = value;
+        return PFS_OK;
+    }
     if (!strcmp(key, "batch_fft_onepass")) {
         if (value != 0 && value != 1) return fail(PFS_ERR_ARG, "batch_fft_onepass must be 0 or 1");
         if (value != c->batch_fft_onepass) {
@@ -1926,6 +1936,10 @@ int pfs_tune_get(pfs_context* c, const char* key, long* value)
     }
     if (!strcmp(key, "batch_fft_onepass")) {
         *value = c->batch_fft_onepass;
+        return PFS_OK;
+    }
+    if (!strcmp(key, "batch_fft_wide")) {
+        *value = c->batch_fft_wide;
         return PFS_OK;
     }
     if (!strcmp(key, "batch_force_shape")) {

@@ -22,11 +22,15 @@ template <> struct CT<float> { typedef float2 C; };
 
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// Products feeding sums are written as explicit fused multiply-adds, so every
+// kernel that runs a line's DFT performs the same roundings whatever the
+// compiler would contract in its context (the kernels that share these
+// helpers give bit-identical results).
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
     C r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
+    r.x = fma(a.x, b.x, -(a.y * b.y));
+    r.y = fma(a.x, b.y, a.y * b.x);
     return r;
 }
 // a * (SIGN * i)
@@ -62,17 +66,20 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft8(C* v)
     C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4<SIGN>(e0, e1, e2, e3);
     dft4<SIGN>(o0, o1, o2, o3);
-    // o_k *= w^k, w = exp(SIGN 2 pi i / 8)
-    C t1, t3;
-    t1.x = (o1.x - SIGN * o1.y) * h;
-    t1.y = (o1.y + SIGN * o1.x) * h;
+    // o_k *= w^k, w = exp(SIGN 2 pi i / 8): o1 w = (o1 + SIGN i o1) h, o3 w^3 = (-o3 + SIGN i o3) h,
+    // each product fused into the sums with e1, e3
+    C a1, a3;
+    a1.x = o1.x - SIGN * o1.y;
+    a1.y = o1.y + SIGN * o1.x;
     C t2 = cmul_i<SIGN>(o2);
-    t3.x = (-o3.x - SIGN * o3.y) * h;
-    t3.y = (-o3.y + SIGN * o3.x) * h;
+    a3.x = -o3.x - SIGN * o3.y;
+    a3.y = -o3.y + SIGN * o3.x;
     v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
-    v[1] = cadd(e1, t1); v[5] = csub(e1, t1);
+    v[1].x = fma(a1.x, h, e1.x); v[1].y = fma(a1.y, h, e1.y);
+    v[5].x = fma(-a1.x, h, e1.x); v[5].y = fma(-a1.y, h, e1.y);
     v[2] = cadd(e2, t2); v[6] = csub(e2, t2);
-    v[3] = cadd(e3, t3); v[7] = csub(e3, t3);
+    v[3].x = fma(a3.x, h, e3.x); v[3].y = fma(a3.y, h, e3.y);
+    v[7].x = fma(-a3.x, h, e3.x); v[7].y = fma(-a3.y, h, e3.y);
 }
 
 template <int R, int SIGN, typename C> __device__ __forceinline__ void dftR(C* v)
@@ -505,6 +512,125 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
     }
 }
 
+// complex64 register kernel with 16-byte global accesses.  Each thread carries
+// two of k_fft_lines_r16's slots: on the row side two adjacent n1 (step A) or
+// two adjacent t (step B), on the tile side two adjacent lines, so every load
+// and store moves two neighbouring complex64 values at once (half the address
+// work per byte of the 8-byte accesses, which keep the texture addresser of
+// the complex64 passes busy 92-95 % of the time).  Every line runs the same
+// operations in the same order as in k_fft_lines_r16: the results are equal
+// bit for bit.  A workgroup whose signal start on either side is not 16-byte
+// aligned moves its values 8 bytes at a time.
+template <int LOGN, int KIND, int SIGN>
+__global__ __launch_bounds__((r16_lines<float, KIND>() * (1 << LOGN) / 32)) void k_fft_lines_r16p(const float2* __restrict__ in,
+                                                                                            float2* __restrict__ out,
+                                                                                            LineMap m,
+                                                                                            const float2* __restrict__ tw,
+                                                                                            float scale)
+{
+    typedef float2 C;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) f4 gf4;
+    constexpr int N = 1 << LOGN;
+    constexpr int TPL = N / 16;
+    constexpr int L = r16_lines<float, KIND>();
+    constexpr int YS = TPL + 1;
+    constexpr int LS = 16 * YS + 1;
+    constexpr int Q = 16 / TPL;
+    constexpr bool ROWIN = KIND == LINES_ROW_IN_TILE_OUT;
+    extern __shared__ __align__(16) unsigned char smem_r16p[];
+    C* lds = reinterpret_cast<C*>(smem_r16p);
+
+    const int tid = threadIdx.x;
+    const long u = blockIdx.y, v0 = (long)blockIdx.x * L, sig = blockIdx.z;
+    const C* src = sig_in(in, m, sig) + u * m.in_su + v0 * m.in_sv;
+    C* dst = sig_out(out, m, sig) + u * m.out_su + v0 * m.out_sv;
+    const bool wide = !(((uintptr_t)src | (uintptr_t)dst) & 15);
+
+    // step A: row side pairs (line la, n1 = 2j, 2j + 1), tile side pairs (lines 2j, 2j + 1, n1)
+    const int la0 = ROWIN ? tid / (TPL / 2) : 2 * (tid % (L / 2));
+    const int n10 = ROWIN ? 2 * (tid % (TPL / 2)) : tid / (L / 2);
+    const int la1 = ROWIN ? la0 : la0 + 1, n11 = ROWIN ? n10 + 1 : n10;
+    C v[2][16];
+    {
+        const unsigned o = (unsigned)(la0 * m.in_sv + n10 * m.in_sn), step = (unsigned)(TPL * m.in_sn);
+        if (wide) {
+#pragma unroll
+            for (int n2 = 0; n2 < 16; ++n2) {
+                const f4 x = __builtin_nontemporal_load((const gf4*)(src + o + n2 * step));
+                v[0][n2].x = x.x;
+                v[0][n2].y = x.y;
+                v[1][n2].x = x.z;
+                v[1][n2].y = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int n2 = 0; n2 < 16; ++n2) {
+                v[0][n2] = gload_nt(src, o + n2 * step);
+                v[1][n2] = gload_nt(src, o + 1 + n2 * step);
+            }
+        }
+    }
+    C w0[16], w1[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+        w0[k2] = gload(tw, n10 * k2);
+        w1[k2] = ROWIN ? gload(tw, n11 * k2) : w0[k2];
+    }
+    dft16_perm<SIGN>(v[0]);
+    dft16_perm<SIGN>(v[1]);
+    {
+        C* ly0 = lds + la0 * LS;
+        C* ly1 = lds + la1 * LS;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            ly0[k2 * YS + n10] = twiddle<SIGN>(v[0][perm_index<16>(k2)], w0[k2]);
+            ly1[k2 * YS + n11] = twiddle<SIGN>(v[1][perm_index<16>(k2)], w1[k2]);
+        }
+    }
+    __syncthreads();
+
+    // step B: tile-side stores pair lines (2j, 2j + 1) at one t, row-side stores pair t = 2j, 2j + 1
+    const int lb0 = ROWIN ? 2 * (tid % (L / 2)) : tid / (TPL / 2);
+    const int t0 = ROWIN ? tid / (L / 2) : 2 * (tid % (TPL / 2));
+    const int lb1 = ROWIN ? lb0 + 1 : lb0, t1 = ROWIN ? t0 : t0 + 1;
+    C x[2][Q][TPL];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int a = 0; a < TPL; ++a) {
+            x[0][q][a] = lds[lb0 * LS + (t0 + TPL * q) * YS + a];
+            x[1][q][a] = lds[lb1 * LS + (t1 + TPL * q) * YS + a];
+        }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        dft_small<TPL, SIGN>(x[0][q]);
+        dft_small<TPL, SIGN>(x[1][q]);
+        const int k2 = t0 + TPL * q;
+        const unsigned o = (unsigned)(lb0 * m.out_sv + k2 * m.out_sn), step = (unsigned)(16 * m.out_sn);
+#pragma unroll
+        for (int k1 = 0; k1 < TPL; ++k1) {
+            C r0 = x[0][q][perm_index<TPL>(k1)], r1 = x[1][q][perm_index<TPL>(k1)];
+            r0.x *= scale;
+            r0.y *= scale;
+            r1.x *= scale;
+            r1.y *= scale;
+            if (wide) {
+                f4 y;
+                y.x = r0.x;
+                y.y = r0.y;
+                y.z = r1.x;
+                y.w = r1.y;
+                if constexpr (r16_nt_store<float, KIND>()) __builtin_nontemporal_store(y, (gf4*)(dst + o + k1 * step));
+                else *(gf4*)(dst + o + k1 * step) = y;
+            } else {
+                gstore<C, r16_nt_store<float, KIND>()>(dst, o + k1 * step, r0);
+                gstore<C, r16_nt_store<float, KIND>()>(dst, o + 1 + k1 * step, r1);
+            }
+        }
+    }
+}
+
 // 512-point lines.  Step A is the register kernel's (32 threads per line, a
 // 16-point DFT each, twiddle, exchange through LDS).  Step B splits each of
 // the 16 DFTs of size 32 over a pair of lanes: lane half h takes the inputs
@@ -600,9 +726,17 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * 32)) void k_fft_lines_r16x2
     }
 }
 
+// The paired complex64 kernel needs the two values a thread moves to be
+// neighbours in memory on both sides: unit element stride along the row side
+// (n) and along the tile side (lines).
+static bool pairs_adjacent(const LineMap& m, int kind)
+{
+    return kind == LINES_ROW_IN_TILE_OUT ? (m.in_sn == 1 && m.out_sv == 1) : (m.in_sv == 1 && m.out_sn == 1);
+}
+
 template <typename T, int LOGN, int KIND, int SIGN>
 static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m,
-                              const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st)
+                              const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st, int paired)
 {
     constexpr int N = 1 << LOGN;
     if (LOGN == 9) {
@@ -619,6 +753,17 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
         dim3 grid(m.nv / L, m.nu, nsig);
         const size_t smem = r16_lds<T, KIND>(LOGN);
         if (smem > lds_limit()) return hipErrorInvalidConfiguration;
+        if constexpr (sizeof(T) == 4) {
+            // wide 2 (the default): the forward-kind passes of 2B <= 128 only.  Measured on
+            // MI355X (per 64 signals): forward passes at 2B = 128 421/434 -> 395/402 us; every
+            // other pass slower with the paired kernel's 124-144 VGPRs (3-4 waves per SIMD
+            // against 7): 2B = 128 inverse 359/388 -> 411/501 us, 2B = 256 forward
+            // 3.04/2.97 -> 3.22/2.96 ms, inverse 3.38/3.10 -> 3.45/3.17 ms.
+            const bool use = paired == 1 || (paired == 2 && KIND == LINES_ROW_IN_TILE_OUT && LOGN <= 7);
+            if (use && pairs_adjacent(m, KIND))
+                return pfs::launch((k_fft_lines_r16p<(LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
+                                   dim3(L * N / 32), smem, st, in, out, m, tw, scale);
+        }
         return pfs::launch((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
                            dim3(L * N / 16), smem, st, in, out, m, tw, scale);
     }
@@ -632,26 +777,27 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
 
 template <typename T, int KIND, int SIGN>
 static hipError_t dispatch_pow2(int logn, const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m,
-                                const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st)
+                                const typename CT<T>::C* tw, T scale, int nsig, hipStream_t st, int paired)
 {
     switch (logn) {
-    case 1: return launch_pow2<T, 1, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 2: return launch_pow2<T, 2, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 3: return launch_pow2<T, 3, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 4: return launch_pow2<T, 4, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 5: return launch_pow2<T, 5, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 6: return launch_pow2<T, 6, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 7: return launch_pow2<T, 7, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 8: return launch_pow2<T, 8, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 9: return launch_pow2<T, 9, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
-    case 10: return launch_pow2<T, 10, KIND, SIGN>(in, out, m, tw, scale, nsig, st);
+    case 1: return launch_pow2<T, 1, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 2: return launch_pow2<T, 2, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 3: return launch_pow2<T, 3, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 4: return launch_pow2<T, 4, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 5: return launch_pow2<T, 5, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 6: return launch_pow2<T, 6, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 7: return launch_pow2<T, 7, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 8: return launch_pow2<T, 8, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 9: return launch_pow2<T, 9, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
+    case 10: return launch_pow2<T, 10, KIND, SIGN>(in, out, m, tw, scale, nsig, st, paired);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
 static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* out, const LineMap& m, int N,
-                               int sign, T scale, const typename CT<T>::C* tw, int kind, int nsig, hipStream_t st)
+                               int sign, T scale, const typename CT<T>::C* tw, int kind, int nsig, hipStream_t st,
+                               int paired = 0)
 {
     if (N < 2 || (N & 1) || m.nv < 1 || m.nu < 1 || nsig < 1) return hipErrorInvalidValue;
     int logn = 0;
@@ -659,10 +805,10 @@ static hipError_t launch_lines(const typename CT<T>::C* in, typename CT<T>::C* o
     if ((1 << logn) == N) {
         if (logn > 10) return hipErrorInvalidValue;
         if (kind == LINES_ROW_IN_TILE_OUT)
-            return sign < 0 ? dispatch_pow2<T, LINES_ROW_IN_TILE_OUT, -1>(logn, in, out, m, tw, scale, nsig, st)
-                            : dispatch_pow2<T, LINES_ROW_IN_TILE_OUT, 1>(logn, in, out, m, tw, scale, nsig, st);
-        return sign < 0 ? dispatch_pow2<T, LINES_TILE_IN_ROW_OUT, -1>(logn, in, out, m, tw, scale, nsig, st)
-                        : dispatch_pow2<T, LINES_TILE_IN_ROW_OUT, 1>(logn, in, out, m, tw, scale, nsig, st);
+            return sign < 0 ? dispatch_pow2<T, LINES_ROW_IN_TILE_OUT, -1>(logn, in, out, m, tw, scale, nsig, st, paired)
+                            : dispatch_pow2<T, LINES_ROW_IN_TILE_OUT, 1>(logn, in, out, m, tw, scale, nsig, st, paired);
+        return sign < 0 ? dispatch_pow2<T, LINES_TILE_IN_ROW_OUT, -1>(logn, in, out, m, tw, scale, nsig, st, paired)
+                        : dispatch_pow2<T, LINES_TILE_IN_ROW_OUT, 1>(logn, in, out, m, tw, scale, nsig, st, paired);
     }
     if (N > 512) return hipErrorInvalidValue;
     int lines = 0;
@@ -862,9 +1008,9 @@ hipError_t launch_fft_lines_f64(const double2* in, double2* out, const LineMap& 
 }
 
 hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m, int N, int sign, float scale,
-                                const float2* tw, int kind, int nsig, hipStream_t st)
+                                const float2* tw, int kind, int nsig, hipStream_t st, int paired)
 {
-    return launch_lines<float>(in, out, m, N, sign, scale, tw, kind, nsig, st);
+    return launch_lines<float>(in, out, m, N, sign, scale, tw, kind, nsig, st, paired);
 }
 
 }  // namespace pfs

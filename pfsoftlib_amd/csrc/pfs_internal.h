@@ -162,8 +162,13 @@ enum LineKind { LINES_ROW_IN_TILE_OUT = 0, LINES_TILE_IN_ROW_OUT = 1 };
 // sign: -1 forward (FFTW_FORWARD), +1 backward.  scale multiplies the result.
 hipError_t launch_fft_lines_f64(const double2* in, double2* out, const LineMap& m, int N, int sign,
                                 double scale, const double2* tw, int kind, int nsig, hipStream_t st);
+// paired: 16-byte accesses (two complex64 values per thread on each side)
+// where the map's row- and tile-side strides are 1, at N = 64, 128, 256;
+// 1 = every such pass, 2 = the forward-kind passes of N <= 128 only, 0 =
+// none.  The results equal the 8-byte kernel's bit for bit.
 hipError_t launch_fft_lines_f32(const float2* in, float2* out, const LineMap& m, int N, int sign,
-                                float scale, const float2* tw, int kind, int nsig, hipStream_t st);
+                                float scale, const float2* tw, int kind, int nsig, hipStream_t st,
+                                int paired = 0);
 // One-pass slice FFT (complex64, N = 64 or 128): both line passes of every
 // slice's 2D FFT in one launch.  sign -1: grids (input side of m) -> pencils
 // G[p][q][k] (output side); sign +1: pencils -> grids, times scale.

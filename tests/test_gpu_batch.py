@@ -163,6 +163,42 @@ def test_chunked_batch_fft_is_bitwise_invariant(ctx, torch, B, nsig):
         assert torch.equal(g, outs[0][0]) and torch.equal(c, outs[0][1])
 
 
+@pytest.mark.parametrize("B,nsig,offset", [(32, 3, 0), (64, 5, 0), (128, 2, 0), (64, 3, 1)])
+def test_wide_fft_passes_match_narrow(ctx, torch, B, nsig, offset):
+    """The complex64 FFT passes with 16-byte accesses (two values per thread,
+    tuning key batch_fft_wide = 1: every pass; the default 2 takes them for
+    the forward passes of 2B <= 128) run every line's operations in the same
+    order as the 8-byte kernel: grids and coefficients equal bit for bit in both
+    directions at 2B = 64, 128, 256.  offset = 1 starts the grids one complex64
+    value past a 16-byte boundary, which the wide kernel moves 8 bytes at a
+    time."""
+    N = 2 * B
+    s = torch.cuda.current_stream()
+    cs = np.concatenate([O.seeded_coeffs(B, 400 + i) for i in range(nsig)]).astype(np.complex64)
+    cdev = torch.from_numpy(cs).cuda()
+    rng = np.random.default_rng(11 + nsig)
+    gin = (rng.uniform(-1, 1, nsig * N ** 3) + 1j * rng.uniform(-1, 1, nsig * N ** 3)).astype(np.complex64)
+    gbig = torch.zeros(nsig * N ** 3 + 1, dtype=torch.complex64, device="cuda")
+    gbig[offset:offset + nsig * N ** 3] = torch.from_numpy(gin).cuda()
+    gdev = gbig[offset:offset + nsig * N ** 3]
+    outs = {}
+    try:
+        for wide in (0, 1):
+            ctx.tune("batch_fft_wide", wide)
+            obig = torch.full((nsig * N ** 3 + 1,), float("nan"), dtype=torch.complex64, device="cuda")
+            g = obig[offset:offset + nsig * N ** 3]
+            ctx.idsoft_batch_f32(B, nsig, cdev, g, stream=s)
+            c = torch.full_like(cdev, float("nan"))
+            ctx.dsoft_batch_f32(B, nsig, gdev, c, stream=s)
+            outs[wide] = (g.clone(), c)
+    finally:
+        ctx.tune("batch_fft_wide", 2)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
+    assert not torch.isnan(outs[1][0]).any() and not torch.isnan(outs[1][1]).any()
+
+
 @pytest.mark.parametrize("B,nsig", [(32, 3), (32, 8), (64, 5), (64, 17)])
 def test_onepass_slice_fft_matches_two_pass(ctx, torch, B, nsig):
     """The one-pass slice FFT (both line passes of a slice's 2D FFT inside one
