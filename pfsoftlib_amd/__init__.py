@@ -357,6 +357,10 @@ class Context:
         """dst <- src (device addresses or tensors), 16- or 8-byte accesses, optionally non-temporal."""
         if width not in (8, 16):
             raise ValueError("copy_dev: width must be 8 or 16 bytes")
+        for t, what in ((dst, "dst"), (src, "src")):
+            if not isinstance(t, int) and (t.device.type != "cuda" or not t.is_contiguous()
+                                           or t.numel() * t.element_size() < nbytes):
+                raise ValueError("copy_dev: %s must be a contiguous device tensor of at least %d bytes" % (what, nbytes))
         flags = (1 if nontemporal else 0) | (2 if width == 8 else 0)
         _check(lib().pfs_copy_dev(self._h, _devptr(dst), _devptr(src), nbytes, flags, _stream_handle(stream)))
 

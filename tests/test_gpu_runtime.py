@@ -30,3 +30,25 @@ def test_library_used_before_torch_keeps_one_hip_runtime():
     r = subprocess.run([sys.executable, "-c", _LIBRARY_FIRST % ROOT], capture_output=True, text=True, timeout=150)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok")
+
+
+@pytest.mark.parametrize("width", [8, 16])
+@pytest.mark.parametrize("nontemporal", [False, True])
+def test_copy_kernel_used_for_counter_calibration(width, nontemporal):
+    # pfs_copy_dev: the 8- and 16-byte copy kernels whose FETCH_SIZE / WRITE_SIZE
+    # calibrate the byte counters must move exactly the bytes asked for
+    import torch
+    import pfsoftlib_amd as P
+    ctx = P.Context(0)
+    try:
+        n = (1 << 20) + 48  # bytes, a multiple of 16 that is not a multiple of the grid stride
+        src = torch.randint(0, 255, (n + 64,), dtype=torch.uint8, device="cuda")
+        dst = torch.zeros(n + 64, dtype=torch.uint8, device="cuda")
+        ctx.copy_dev(dst, src, n, nontemporal=nontemporal, width=width)
+        torch.cuda.synchronize()
+        assert torch.equal(dst[:n], src[:n])
+        assert not dst[n:].any()  # nothing past the end
+        with pytest.raises(ValueError):
+            ctx.copy_dev(dst, src, n, width=4)
+    finally:
+        ctx.close()
