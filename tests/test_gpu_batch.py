@@ -345,3 +345,30 @@ def test_mixed_bandwidth_graph_replays_match_direct_launches(ctx, torch):
         ctx.tune("batch_graphs", 1)
         for g, d in zip(gd, direct):
             assert torch.equal(g, d), rnd
+
+
+@pytest.mark.parametrize("key,value", [("batch_fft_onepass", 1), ("batch_fft_wide", 1), ("batch_fft_wide", 0)])
+def test_mixed_bandwidth_batch_under_fft_keys(ctx, torch, key, value):
+    """The mixed-bandwidth entry points (per-signal address tables, graph
+    capture per composition) with the FFT kernels a tuning key selects: the
+    one-pass slice kernel, the 16-byte kernel in every pass, the 8-byte
+    kernel only.  Every grid and coefficient equals the default's bit for bit."""
+    s = torch.cuda.current_stream()
+    cs, cd, gd0 = _mix_inputs(torch, 7300)
+    default = ctx.tune_get(key)
+    outs = []
+    try:
+        for v in (default, value):
+            ctx.tune(key, v)
+            gd = [torch.full_like(g, float("nan")) for g in gd0]
+            ctx.idsoft_batch_multi(MIX, cd, gd, stream=s)
+            back = [torch.full_like(c, float("nan")) for c in cd]
+            ctx.dsoft_batch_multi(MIX, gd, back, stream=s)
+            outs.append((gd, back))
+    finally:
+        ctx.tune(key, default)
+    torch.cuda.synchronize()
+    for i in range(len(MIX)):
+        assert torch.equal(outs[0][0][i], outs[1][0][i]), ("inverse", i)
+        assert torch.equal(outs[0][1][i], outs[1][1][i]), ("forward", i)
+        assert not torch.isnan(outs[1][1][i]).any()
