@@ -82,6 +82,18 @@ __global__ void k_plan_nodes(int B, double* cosb, double* sinh, double* cosh, do
     tw32[k] = tf;
 }
 
+// pws[e][k] = sin(beta_k / 2)^e (e < B), pwc[e][k] = cos(beta_k / 2)^e
+// (e < 2B - 1): the base-row powers of every canonical pair, made once per
+// bandwidth instead of in every transform.
+__global__ void k_plan_powers(int B, const double* sinh, const double* cosh, double* pws, double* pwc)
+{
+    const int N = 2 * B, e = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    if (e < B) pws[(long)e * N + k] = ipow(sinh[k], e);
+    if (e < 2 * B - 1) pwc[(long)e * N + k] = ipow(cosh[k], e);
+}
+
 __global__ void k_plan_normfac(int B, double* nf)
 {
     int M = blockIdx.x, Mp = threadIdx.x;
@@ -460,7 +472,9 @@ int get_plan(pfs_context* c, int B, Plan** out)
     size_t off_wpairs = off_xpairs + align(xpairs.size() * sizeof(int2));
     size_t off_tw = off_wpairs + align(wpairs.size() * sizeof(int2));
     size_t off_tw32 = off_tw + align(N * sizeof(double2));
-    size_t total = off_tw32 + align(N * sizeof(float2));
+    size_t off_pws = off_tw32 + align(N * sizeof(float2));
+    size_t off_pwc = off_pws + align((size_t)B * N * sizeof(double));
+    size_t total = off_pwc + align((size_t)(2 * B - 1) * N * sizeof(double));
 
     Plan p;
     p.B = B;
@@ -482,6 +496,8 @@ int get_plan(pfs_context* c, int B, Plan** out)
     p.xorder[1] = (int2*)(base + off_wpairs);
     p.t.tw = (double2*)(base + off_tw);
     p.t.tw32 = (float2*)(base + off_tw32);
+    p.t.pws = (double*)(base + off_pws);
+    p.t.pwc = (double*)(base + off_pwc);
     hipError_t e = hipMemcpyAsync((void*)p.t.pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   c->stream);
     if (e == hipSuccess)
@@ -494,6 +510,9 @@ int get_plan(pfs_context* c, int B, Plan** out)
         e = pfs::launch(k_plan_nodes, dim3((N + 127) / 128), dim3(128), 0, c->stream, B, (double*)p.t.cosb,
                         (double*)p.t.sinh, (double*)p.t.cosh, (double*)p.t.wts, (double2*)p.t.tw, (float2*)p.t.tw32);
     if (e == hipSuccess) e = pfs::launch(k_plan_normfac, dim3(B), dim3(64), 0, c->stream, B, (double*)p.t.normfac);
+    if (e == hipSuccess)
+        e = pfs::launch(k_plan_powers, dim3((N + 127) / 128, 2 * B - 1), dim3(128), 0, c->stream, B, p.t.sinh, p.t.cosh,
+                        (double*)p.t.pws, (double*)p.t.pwc);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         ctx_free(c, p.mem, p.bytes);

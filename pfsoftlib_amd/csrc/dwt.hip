@@ -73,17 +73,6 @@ __device__ __forceinline__ CaseInfo case_info(int c, int M, int Mp, int N)
     return ci;
 }
 
-__device__ __forceinline__ double ipow(double x, int e)
-{
-    double r = 1.0;
-    while (e) {
-        if (e & 1) r *= x;
-        x *= x;
-        e >>= 1;
-    }
-    return r;
-}
-
 // Recurrence coefficients for the step from degree l = M + r to l + 1
 // (fn_dwt.hpp:167-192), FP64: (f1, f1*f2, c1), zero from the last row on so
 // the padded rows stay 0.
@@ -128,35 +117,41 @@ struct Chains {
     double cb[NCH], cur[NCH], prev[NCH];
 };
 
-// Loads the node tables of every chain; separate from init_chains so the
-// loads can all be in flight before any of them is used.
+// Loads the node tables of every chain for the pair (M, M'): cos(beta), the
+// base-row powers sin(beta/2)^(M-M') and cos(beta/2)^(M+M') from the plan's
+// power tables, and the weight.  Separate from init_chains so the loads can
+// all be in flight before any of them is used.
 template <int NCH>
 struct ChainTables {
-    double cb[NCH], sh[NCH], chh[NCH], w[NCH];
+    double cb[NCH], ps[NCH], pc[NCH], w[NCH];
 };
 
 template <int NCH>
-__device__ __forceinline__ void load_chain_tables(ChainTables<NCH>& ct, const PlanTables& t, int kbase, bool weighted)
+__device__ __forceinline__ void load_chain_tables(ChainTables<NCH>& ct, const PlanTables& t, int kbase, bool weighted,
+                                                  int M, int Mp)
 {
+    const double* ps = t.pws + (M - Mp) * t.N;
+    const double* pc = t.pwc + (M + Mp) * t.N;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const int k = min(kbase + 64 * c, t.N - 1);
         ct.cb[c] = t.cosb[k];
-        ct.sh[c] = t.sinh[k];
-        ct.chh[c] = t.cosh[k];
+        ct.ps[c] = ps[k];
+        ct.pc[c] = pc[k];
         ct.w[c] = weighted ? t.wts[k] : 1.0;
     }
 }
 
+// Base row d^M_{M M'}(beta) = nf sin(beta/2)^(M-M') cos(beta/2)^(M+M')
+// (times the weight for the forward), fn_dwt.hpp:152-160.
 template <int NCH>
-__device__ __forceinline__ void init_chains(Chains<NCH>& ch, const ChainTables<NCH>& ct, double nf, int M, int Mp,
-                                            int kbase, int N)
+__device__ __forceinline__ void init_chains(Chains<NCH>& ch, const ChainTables<NCH>& ct, double nf, int kbase, int N)
 {
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const bool kv = kbase + 64 * c < N;
         ch.cb[c] = kv ? ct.cb[c] : 0.0;
-        ch.cur[c] = kv ? nf * ipow(ct.sh[c], M - Mp) * ipow(ct.chh[c], M + Mp) * ct.w[c] : 0.0;
+        ch.cur[c] = kv ? nf * ct.ps[c] * ct.pc[c] * ct.w[c] : 0.0;
         ch.prev[c] = 0.0;
     }
 }
@@ -339,6 +334,29 @@ struct LaneCase {
     double sgn_odd;  // sign for odd rows (differs for the reversed cases)
 };
 
+// Packed indices of the coefficients (l, Mc, Mpc) for l = l0, l0 + 4, l0 + 8,
+// ...: for fixed orders the index is a cubic polynomial in l, so it advances
+// by forward differences, three integer additions a step in place of the
+// multiplications of coeff_index32 (exact while l < 512).
+struct IdxWalk {
+    int f, d1, d2, d3;
+    __device__ __forceinline__ IdxWalk(int l0, int Mc, int Mpc)
+    {
+        const int f0 = coeff_index32(l0, Mc, Mpc), f1 = coeff_index32(l0 + 4, Mc, Mpc);
+        const int f2 = coeff_index32(l0 + 8, Mc, Mpc), f3 = coeff_index32(l0 + 12, Mc, Mpc);
+        f = f0;
+        d1 = f1 - f0;
+        d2 = f2 - 2 * f1 + f0;
+        d3 = f3 - 3 * f2 + 3 * f1 - f0;
+    }
+    __device__ __forceinline__ void step()
+    {
+        f += d1;
+        d1 += d2;
+        d2 += d3;
+    }
+};
+
 __device__ __forceinline__ LaneCase lane_case(int c, int M, int Mp, int N, long* vec)
 {
     CaseInfo ci = case_info(c, M, Mp, N);
@@ -383,7 +401,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
     const int j = lane & 15, kk = lane >> 4;
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
     ChainTables<NCH> ct;
-    load_chain_tables<NCH>(ct, t, kw + lane, true);
+    load_chain_tables<NCH>(ct, t, kw + lane, true, M, Mp);
     const double nf = t.normfac[M * B + Mp];
     // B operands: lane (j + 16 kk) holds S[kw + 4m + kk][j], j = 2 case + re/im
     double sreg[NCH * 16];
@@ -419,7 +437,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
     Chains<NCH> ch;
-    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+    init_chains<NCH>(ch, ct, nf, kw + lane, N);
     if (SMALL) {
 #pragma unroll
         for (int m = 0; m < NCH * 16; ++m)
@@ -517,13 +535,12 @@ __device__ __forceinline__ void fwd_nb_pair(const PlanTables& t, const double2* 
     const int lane = threadIdx.x & 63, j = lane & 15, kk = lane >> 4;
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
     long cvec[8];
-    int cflip = 0, cact = 0;
+    int cflip = 0;
 #pragma unroll
     for (int cs = 0; cs < 8; ++cs) {
         const CaseInfo ci = case_info(cs, M, Mp, N);
         cvec[cs] = ci.vec;
         cflip |= ci.flip << cs;
-        cact |= ci.active << cs;
     }
     // block b of case cs: node n = 64 b + lane, reversed for the d(pi - beta) cases
     auto load_block = [&](int b, double2 (&raw)[8]) {
@@ -533,8 +550,6 @@ __device__ __forceinline__ void fwd_nb_pair(const PlanTables& t, const double2* 
     };
     double2 raw[8];
     load_block(0, raw);
-    ChainTables<1> ct;
-    load_chain_tables<1>(ct, t, lane, true);
     const double nf = t.normfac[M * B + Mp];
     for (int r = lane; r < RP; r += 64) recurrence_coeffs(M, Mp, r, R, cf, RP);
 
@@ -543,27 +558,24 @@ __device__ __forceinline__ void fwd_nb_pair(const PlanTables& t, const double2* 
     for (int c = 0; c < NCHK; ++c) acc[c] = (v4d){0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
     for (int b = 0; b < NB; ++b) {
+        // this block's node tables (small, L2-resident), loaded here rather
+        // than a block ahead so that they hold no registers across the chunks
+        ChainTables<1> ct;
+        load_chain_tables<1>(ct, t, 64 * b + lane, true, M, Mp);
         wave_sync();  // step coefficients written / the previous block's tile reads done
+        // an inactive case reads a valid pencil (another case's): its output
+        // columns are computed like the others and never stored
 #pragma unroll
-        for (int cs = 0; cs < 8; ++cs) {
-            const bool on = (cact >> cs) & 1;
-            double2 v;
-            v.x = on ? raw[cs].x : 0.0;
-            v.y = on ? raw[cs].y : 0.0;
-            *reinterpret_cast<double2*>(tile + cs * FSTG_CASE + 2 * lane) = v;
-        }
+        for (int cs = 0; cs < 8; ++cs) *reinterpret_cast<double2*>(tile + cs * FSTG_CASE + 2 * lane) = raw[cs];
         wave_sync();
         // B operands: lane (j + 16 kk) holds S[64 b + 4 m + kk][j], j = 2 case + re/im
         double sreg[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) sreg[m] = tile[(j >> 1) * FSTG_CASE + 2 * (4 * m + kk) + (j & 1)];
         Chains<1> ch;
-        init_chains<1>(ch, ct, nf, M, Mp, 64 * b + lane, N);
-        // the next block's loads stay in flight across this block's chunks
-        if (b + 1 < NB) {
-            load_block(b + 1, raw);
-            load_chain_tables<1>(ct, t, 64 * (b + 1) + lane, true);
-        }
+        init_chains<1>(ch, ct, nf, 64 * b + lane, N);
+        // the next block's pencils stay in flight across this block's chunks
+        if (b + 1 < NB) load_block(b + 1, raw);
         wave_sync();  // operand reads done before the recurrence writes the tile
 #pragma unroll
         for (int chk = 0; chk < NCHK; ++chk) {
@@ -581,16 +593,19 @@ __device__ __forceinline__ void fwd_nb_pair(const PlanTables& t, const double2* 
         }
     }
     if (!lc.active) return;
-    const int orow = lane >> 4;
+    // this lane's rows r = 16 chk + 4 v + orow: degrees M + orow + 4 (4 chk + v),
+    // all of the parity of orow (norm * sign is exact, sign = +-1)
+    int orow = lane >> 4;
+    asm volatile("" : "+v"(orow));  // the walk's set-up stays here, out of the loop's registers
+    const double ns = norm * ((orow & 1) ? lc.sgn_odd : lc.sgn);
+    double* cdst = reinterpret_cast<double*>(coeffs) + (j & 1);
+    IdxWalk iw(M + orow, lc.Mc, lc.Mpc);
 #pragma unroll
     for (int chk = 0; chk < NCHK; ++chk)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int r = chk * 16 + 4 * v + orow;
-            if (r < R) {
-                const double sg = (r & 1) ? lc.sgn_odd : lc.sgn;
-                reinterpret_cast<double*>(coeffs + coeff_index32(M + r, lc.Mc, lc.Mpc))[j & 1] = norm * (sg * acc[chk][v]);
-            }
+            if (chk * 16 + 4 * v + orow < R) cdst[2 * iw.f] = ns * acc[chk][v];
+            iw.step();
         }
 }
 
@@ -671,7 +686,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
 
     const int kw = w * 64 * NCH;
     ChainTables<NCH> ct;
-    load_chain_tables<NCH>(ct, t, kw + lane, false);
+    load_chain_tables<NCH>(ct, t, kw + lane, false, M, Mp);
     const double nf = t.normfac[M * B + Mp];
 
     // B operand rows: lane (j + 16 b) holds n' * sign * coeff(l = M + r0 + 4s + b, case j/2).(re|im)
@@ -697,7 +712,7 @@ __global__ __launch_bounds__(64 * WPP) __attribute__((amdgpu_waves_per_eu(2))) v
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
     Chains<NCH> ch;
-    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+    init_chains<NCH>(ch, ct, nf, kw + lane, N);
 
     v4d acc[NCH * 4];
 #pragma unroll
@@ -823,7 +838,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
     double* cf = sm;
 
     ChainTables<NCH> ct;
-    load_chain_tables<NCH>(ct, t, kw + lane, false);
+    load_chain_tables<NCH>(ct, t, kw + lane, false, M, Mp);
     const double nf = t.normfac[M * B + Mp];
 
     // this lane's right-hand side j = lane % 16: case j/2, part j&1
@@ -839,7 +854,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
 
     for (int r = tid; r < RP; r += 64 * WPP) recurrence_coeffs(M, Mp, r, R, cf, RP);
     Chains<NCH> ch;
-    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+    init_chains<NCH>(ch, ct, nf, kw + lane, N);
 
     double acc[NCH][QW];
 #pragma unroll
@@ -956,7 +971,7 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
     double* stg = sm + 4 * RP + 256 * w;         // this wave's 16 degrees x 16 values
 
     ChainTables<NCH> ct;
-    load_chain_tables<NCH>(ct, t, kw + lane, false);
+    load_chain_tables<NCH>(ct, t, kw + lane, false, M, Mp);
     const double nf = t.normfac[M * B + Mp];
 
     // the right-hand side this lane broadcasts (j = lane % 16: case j/2,
@@ -966,14 +981,20 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
     const LaneCase lc = lane_case(j >> 1, M, Mp, N, nullptr);
     const double* csrc = reinterpret_cast<const double*>(coeffs) + (j & 1);
     const double fac = lc.active ? norm * ((dl & 1) ? lc.sgn_odd : lc.sgn) : 0.0;
-    auto cidx = [&](int r) { return r < R ? 2 * coeff_index32(M + r, lc.Mc, lc.Mpc) : 0; };
+    // the lane's loads walk degrees M + dl + 4 t in order (t = 4 chunk + i)
+    IdxWalk iw(M + dl, lc.Mc, lc.Mpc);
+    auto cidx = [&](int r) {
+        const int x = r < R ? 2 * iw.f : 0;
+        iw.step();
+        return x;
+    };
     double nxt[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) nxt[i] = csrc[cidx(dl + 4 * i)];
 
     recurrence_coeffs4(M, Mp, R, RP, cf4, tid, 64 * WPP);
     Chains<NCH> ch;
-    init_chains<NCH>(ch, ct, nf, M, Mp, kw + lane, N);
+    init_chains<NCH>(ch, ct, nf, kw + lane, N);
 
     double acc[NCH][16];
 #pragma unroll

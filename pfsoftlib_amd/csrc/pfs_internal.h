@@ -107,6 +107,19 @@ __device__ __forceinline__ int coeff_index32(int l, int M, int Mp)
     return (4 * l * l * l - l) / 3 + iM + n * iMp;
 }
 
+// x^e by binary powering (e >= 0).  The plan's power tables are made with
+// it, so a table entry equals the value a kernel would compute in place.
+__host__ __device__ __forceinline__ double ipow(double x, int e)
+{
+    double r = 1.0;
+    while (e) {
+        if (e & 1) r *= x;
+        x *= x;
+        e >>= 1;
+    }
+    return r;
+}
+
 // Device-resident tables for one bandwidth (all FP64 unless noted).
 struct PlanTables {
     int B, N;
@@ -115,6 +128,8 @@ struct PlanTables {
     const double* sinh;    // sin(beta_k / 2)
     const double* cosh;    // cos(beta_k / 2)
     const double* wts;     // quadrature weights w_B(k)
+    const double* pws;     // sin(beta_k / 2)^e, [e * N + k], e < B (ipow)
+    const double* pwc;     // cos(beta_k / 2)^e, [e * N + k], e < 2B - 1 (ipow)
     const double* normfac; // sqrt((2M+1)/2) * sqrt(C(2M, M-M')), [M*B + M'], M >= M' >= 0
     const int2* pairs;     // canonical (M, M'), M >= M' >= 0, heaviest first
     const int2* xpairs;    // the same pairs in the FP64 DWT launch order (workgroup b runs xpairs[b])
