@@ -419,9 +419,14 @@ struct InvStage {
     float f[4];
 };
 
-template <int SPW>
-__device__ __forceinline__ void inv_gather(InvStage<SPW>& S, const float* const (&cs)[SPW], const BCase& cb, int M,
-                                           int R, long cstride, float norm, int kk, int q)
+// PAIRED (interleaved layout with an even row length, SPW even): signals
+// s0 + sp and s0 + sp + 1 are adjacent complex values, so one 16-byte load
+// from cp[sp / 2] gives both and the lane keeps its part of each -- half the
+// gather instructions, each covering the same cache lines.
+template <int SPW, bool PAIRED>
+__device__ __forceinline__ void inv_gather(InvStage<SPW>& S, const float* const (&cs)[SPW], const float* const (&cp)[SPW],
+                                           const BCase& cb, int M, int R, long cstride, float norm, int kk, int q,
+                                           int part)
 {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -429,8 +434,17 @@ __device__ __forceinline__ void inv_gather(InvStage<SPW>& S, const float* const 
         const bool live = cb.active && r < R;
         const long ci = live ? 2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc) : 0;
         S.f[u] = live ? norm * (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn) : 0.0f;
+        if constexpr (PAIRED) {
 #pragma unroll
-        for (int sp = 0; sp < SPW; ++sp) S.v[sp][u] = cs[sp][ci];
+            for (int sp = 0; sp < SPW; sp += 2) {
+                const float4 v = *reinterpret_cast<const float4*>(cp[sp] + ci);
+                S.v[sp][u] = part ? v.y : v.x;
+                S.v[sp + 1][u] = part ? v.w : v.z;
+            }
+        } else {
+#pragma unroll
+            for (int sp = 0; sp < SPW; ++sp) S.v[sp][u] = cs[sp][ci];
+        }
     }
 }
 
@@ -518,11 +532,11 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
 // gathered per lane.  The 8 beta-vectors of each signal are staged per wave
 // in LDS and written out as contiguous pencils.
 // One work item of the inverse contraction; smf: WPB x NN x OSTRF floats of LDS.
-template <int NN, int WPB, int SPW, int PREC>
-__device__ __forceinline__ void inv_item(const PlanTables& t, const long* __restrict__ roff,
-                                         const float* __restrict__ Ti, const SplitTab& ts,
-                                         const float2* __restrict__ coeffs, float2* __restrict__ G, int nsig,
-                                         long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf)
+template <int NN, int WPB, int SPW, int PREC, bool PAIRED>
+__device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* __restrict__ roff,
+                                              const float* __restrict__ Ti, const SplitTab& ts,
+                                              const float2* __restrict__ coeffs, float2* __restrict__ G, int nsig,
+                                              long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf)
 {
     constexpr int NKB = NN / 16;
     if (bw.pi >= t.npairs) return;
@@ -536,9 +550,12 @@ __device__ __forceinline__ void inv_item(const PlanTables& t, const long* __rest
     const long N3 = (long)NN * NN * NN;
 
     const float* cs[SPW];
+    const float* cp[SPW];  // PAIRED: the even signal of each pair (a valid one past nsig); row length even
 #pragma unroll
-    for (int sp = 0; sp < SPW; ++sp)
+    for (int sp = 0; sp < SPW; ++sp) {
         cs[sp] = reinterpret_cast<const float*>(coeffs + (long)min(s0 + sp, nsig - 1) * cdist) + part;
+        cp[sp] = reinterpret_cast<const float*>(coeffs + ((s0 + (sp & ~1)) < nsig ? s0 + (sp & ~1) : 0));
+    }
     const long toff = (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
     const float* ti = Ti + toff;
 
@@ -556,10 +573,10 @@ __device__ __forceinline__ void inv_item(const PlanTables& t, const long* __rest
     InvTab<NKB, PREC> tab;
 #pragma unroll
     for (int kb = 0; kb < NKB; ++kb) inv_tab_load<NKB, PREC>(tab, kb, ti, ts, toff);
-    inv_gather<SPW>(bc, cs, cb, M, R, cstride, norm, kk, 0);
+    inv_gather<SPW, PAIRED>(bc, cs, cp, cb, M, R, cstride, norm, kk, 0, part);
     for (int q = 0; q < nch; ++q) {
         const int qn = q + 1 < nch ? q + 1 : q;
-        inv_gather<SPW>(bn, cs, cb, M, R, cstride, norm, kk, qn);
+        inv_gather<SPW, PAIRED>(bn, cs, cp, cb, M, R, cstride, norm, kk, qn, part);
         __builtin_amdgcn_sched_barrier(0);
         inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
         __builtin_amdgcn_sched_barrier(0);
@@ -593,6 +610,27 @@ __device__ __forceinline__ void inv_item(const PlanTables& t, const long* __rest
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// The paired gathers need the interleaved layout (cdist 1) with an even row
+// length, so 16-byte loads stay aligned and inside the row.  Measured per 64
+// signals (interleaved A/B, two runs each, same results): 2B = 128 inverse
+// 3.30 -> 3.01-3.03 ms (512 x B = 64: 25.0 k -> 25.4 k pairs/s); 2B = 256
+// 3.61 -> 3.66-3.67 ms, so 2B = 256 keeps the 8-byte gathers.
+template <int NN, int WPB, int SPW, int PREC>
+__device__ __forceinline__ void inv_item(const PlanTables& t, const long* __restrict__ roff,
+                                         const float* __restrict__ Ti, const SplitTab& ts,
+                                         const float2* __restrict__ coeffs, float2* __restrict__ G, int nsig,
+                                         long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf)
+{
+    if constexpr (SPW % 2 == 0 && NN <= 128) {
+        if (cdist == 1 && (cstride & 1) == 0) {
+            inv_item_body<NN, WPB, SPW, PREC, true>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw,
+                                                    smf);
+            return;
+        }
+    }
+    inv_item_body<NN, WPB, SPW, PREC, false>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw, smf);
 }
 
 template <int NN, int WPB, int SPW, int PREC>
