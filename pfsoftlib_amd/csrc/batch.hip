@@ -1,8 +1,9 @@
 // Batched mixed-precision transforms: many complex64 signals of one bandwidth.
 //
 // Precision: the Wigner-d tables come from the FP64 three-term recurrence
-// (as in dwt.hip) and are stored once per bandwidth in FP32; the quadrature
-// weights and FFT twiddles are FP64-computed and rounded to FP32; the slice
+// (as in dwt.hip) and are stored once per bandwidth in FP32, the forward's
+// multiplied by the FP64 quadrature weights before rounding; the FFT
+// twiddles are FP64-computed and rounded to FP32; the slice
 // FFTs and the DWT contraction run in FP32 (v_mfma_f32_16x16x4_f32, FP32
 // accumulation).
 //
@@ -117,14 +118,14 @@ __device__ __forceinline__ int out_row(int lane, int v, int layout)
 }
 
 // FP32 tables for pair index pi, rows r < RP(pi) (16-padded, zero beyond R),
-// unweighted, positive base, from the FP64 recurrence.  Both tables are cut
+// positive base, from the FP64 recurrence: Tf holds d * w_k (the forward's
+// weighted table, the product rounded once), Ti holds d.  Both tables are cut
 // into 16 x 16 tiles; tile (q, kb) of pair pi (rows 16q.., betas 16kb..) sits
 // at tile index (roff[pi]/16 + q) * N/16 + kb.  Tf stores a tile row-major
 // (r, k) -> 16 r + k, Ti column-major (r, k) -> 16 k + r, so that a lane of
-// either contraction reads its four operands as one float4.  Block 0 also
-// writes the FP32 quadrature weights.
-__global__ void k_wigner_tiles_f32(PlanTables t, const long* __restrict__ roff, float* __restrict__ wts32,
-                                   float* __restrict__ Tf, float* __restrict__ Ti)
+// either contraction reads its four operands as one float4.
+__global__ void k_wigner_tiles_f32(PlanTables t, const long* __restrict__ roff, float* __restrict__ Tf,
+                                   float* __restrict__ Ti)
 {
     const int N = t.N, B = t.B, NKB = N >> 4;
     const int2 pr = t.pairs[blockIdx.x];
@@ -132,16 +133,15 @@ __global__ void k_wigner_tiles_f32(PlanTables t, const long* __restrict__ roff, 
     const long tile0 = (roff[blockIdx.x] >> 4) * NKB;
     const double nf = t.normfac[M * B + Mp];
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
-        if (blockIdx.x == 0) wts32[k] = (float)t.wts[k];
         double d = nf, dp = 0.0;
         const double sh = t.sinh[k], ch = t.cosh[k], cb = t.cosb[k];
         for (int e = 0; e < M - Mp; ++e) d *= sh;
         for (int e = 0; e < M + Mp; ++e) d *= ch;
+        const double wk = t.wts[k];
         for (int r = 0; r < RP; ++r) {
-            const float val = r < R ? (float)d : 0.0f;
             const long tb = (tile0 + (long)(r >> 4) * NKB + (k >> 4)) * 256;
-            Tf[tb + 16 * (r & 15) + (k & 15)] = val;
-            Ti[tb + 16 * (k & 15) + (r & 15)] = val;
+            Tf[tb + 16 * (r & 15) + (k & 15)] = r < R ? (float)(d * wk) : 0.0f;
+            Ti[tb + 16 * (k & 15) + (r & 15)] = r < R ? (float)d : 0.0f;
             if (r >= R - 1) continue;
             const double idx = M + r;
             const double nrm = sqrt((2.0 * idx + 3.0) / (2.0 * idx + 1.0));
@@ -180,19 +180,18 @@ __device__ __forceinline__ BlockWork block_work(int nsg)
 }
 
 // Forward contraction of one canonical pair for SPW signals per wave:
-// C[r][j] = sum_k D[r][k] * w_k * G_s[pencil(j/2)][k or N-1-k].(re|im by j&1),
-// as v_mfma_f32_16x16x4f32 over NC 16-degree chunks (A = table tile rows, read
-// from the row-major tiles) and 16-beta blocks (B = the signal's 8 case
+// C[r][j] = sum_k (D[r][k] w_k) * G_s[pencil(j/2)][k or N-1-k].(re|im by j&1),
+// as v_mfma_f32_16x16x4f32 over NC 16-degree chunks (A = weighted table tile
+// rows, read from the row-major tiles) and 16-beta blocks (B = the signal's 8 case
 // vectors, read straight from the pencils).  Within a 16-beta block, MFMA
 // step t contracts betas 16 kb + 4 kk + t over the lane groups kk.  The loads
 // of block kb + 1 are issued before the MFMAs of block kb (two register
 // stages), so memory latency overlaps the contraction.
 template <int NN, int SPW, int NC, int PREC = 0>
 struct FwdStage {
-    float4 a[PREC ? 1 : NC];                      // FP32 table operands
+    float4 a[PREC ? 1 : NC];                      // FP32 table operands (weighted)
     v4s as[PREC ? NC : 1][split_parts(PREC)];     // split table operands
     float4 g[SPW];  // two consecutive complex values of this lane's case pencil
-    float4 w;
 };
 
 // Exchange with the partner lane (lane ^ 1) through DPP quad_perm [1,0,3,2].
@@ -215,7 +214,7 @@ struct SplitTab {
 
 template <int NN, int SPW, int NC, int PREC>
 __device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC, PREC>& S, const float* const (&gp)[SPW], const float* tf,
-                                         const SplitTab& ts, long toff, const float* wl, int e0, int de, int kb)
+                                         const SplitTab& ts, long toff, int e0, int de, int kb)
 {
     constexpr int NKB = NN / 16;
     if constexpr (PREC == 0) {
@@ -237,7 +236,6 @@ __device__ __forceinline__ void fwd_load(FwdStage<NN, SPW, NC, PREC>& S, const f
         const v4f g = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(gp[sp] + e));
         S.g[sp] = make_float4(g.x, g.y, g.z, g.w);
     }
-    S.w = *reinterpret_cast<const float4*>(wl + 16 * kb);
 }
 
 template <int NN, int SPW, int NC, int PREC>
@@ -255,10 +253,10 @@ __device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN,
         // direct:   even (o.x, o.z, x.x, x.z), odd (x.y, x.w, o.y, o.w)
         // reversed: even (o.z, o.x, x.z, x.x), odd (x.w, x.y, o.w, o.y)
         const float d0 = odd ? x.y : o.x, d1 = odd ? x.w : o.z, d2 = odd ? o.y : x.x, d3 = odd ? o.w : x.z;
-        b[sp][0] = (flip ? d1 : d0) * S.w.x;
-        b[sp][1] = (flip ? d0 : d1) * S.w.y;
-        b[sp][2] = (flip ? d3 : d2) * S.w.z;
-        b[sp][3] = (flip ? d2 : d3) * S.w.w;
+        b[sp][0] = flip ? d1 : d0;
+        b[sp][1] = flip ? d0 : d1;
+        b[sp][2] = flip ? d3 : d2;
+        b[sp][3] = flip ? d2 : d3;
     }
     if constexpr (PREC == 0) {
 #pragma unroll
@@ -284,7 +282,7 @@ __device__ __forceinline__ void fwd_mfma(v4f (&acc)[NC][SPW], const FwdStage<NN,
 
 template <int NN, int SPW, int NC, int PREC>
 __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const float* tf, const SplitTab& ts, long toff,
-                                         const float* wl, int e0, int de, float2* __restrict__ coeffs, const BCase& cb,
+                                         int e0, int de, float2* __restrict__ coeffs, const BCase& cb,
                                          int M, int R, int s0, int nsig, long cdist, long cstride, float norm, int lane,
                                          int layout)
 {
@@ -303,9 +301,9 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
     constexpr int ST = ST0 < NKB ? ST0 : NKB;
     FwdStage<NN, SPW, NC, PREC> S[ST];
 #pragma unroll
-    for (int st = 0; st + 1 < ST; ++st) fwd_load(S[st], gp, tf, ts, toff, wl, e0, de, st);
+    for (int st = 0; st + 1 < ST; ++st) fwd_load(S[st], gp, tf, ts, toff, e0, de, st);
     if constexpr (ST == 1) {
-        fwd_load(S[0], gp, tf, ts, toff, wl, e0, de, 0);
+        fwd_load(S[0], gp, tf, ts, toff, e0, de, 0);
         fwd_mfma(acc, S[0], odd, flip);
     } else {
         // The scheduling barriers keep each stage's loads ahead of the other
@@ -317,7 +315,7 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
                 // branch here would make the compiler drain every outstanding
                 // load before the next MFMAs
                 const int nk = kb + st + ST - 1;
-                fwd_load(S[(st + ST - 1) % ST], gp, tf, ts, toff, wl, e0, de, nk < NKB ? nk : NKB - 1);
+                fwd_load(S[(st + ST - 1) % ST], gp, tf, ts, toff, e0, de, nk < NKB ? nk : NKB - 1);
                 __builtin_amdgcn_sched_barrier(0);
                 fwd_mfma(acc, S[st], odd, flip);
                 __builtin_amdgcn_sched_barrier(0);
@@ -325,8 +323,34 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
         }
     }
 
-    if (!cb.active) return;
+    if (!cb.active) return;  // lanes 2c, 2c+1 (case c, re / im) leave together
     const int part = lane & 1;
+    if (SPW % 2 == 0 && cdist == 1 && (cstride & 1) == 0) {
+        // interleaved layout, even row length: signals s0 + sp, s0 + sp + 1 of
+        // a coefficient are adjacent complex values, written by the re lane as
+        // one 16-byte store after taking the im values from its partner (a slot
+        // past nsig is row padding, never read back)
+#pragma unroll
+        for (int sp = 0; sp < SPW; sp += 2) {
+            if (s0 + sp >= nsig) break;
+            float* cp = reinterpret_cast<float*>(coeffs + (s0 + sp));
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = ch * 16 + out_row(lane, v, layout);  // the same in both lanes of a pair
+                    if (r < R) {
+                        const float sg = (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
+                        const float a0 = norm * (sg * acc[ch][sp][v]), a1 = norm * (sg * acc[ch][sp + 1][v]);
+                        const float b0 = swap_pair(a0), b1 = swap_pair(a1);
+                        if (!part)
+                            *reinterpret_cast<float4*>(cp + 2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc)) =
+                                make_float4(a0, b0, a1, b1);
+                    }
+                }
+        }
+        return;
+    }
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp) {
         if (s0 + sp >= nsig) break;
@@ -347,16 +371,16 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
 // Run fwd_pair with the pair's chunk count as a template argument.
 template <int NN, int SPW, int NC, int PREC>
 __device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[SPW], const float* tf, const SplitTab& ts,
-                                             long toff, const float* wl, int e0, int de, float2* __restrict__ coeffs,
+                                             long toff, int e0, int de, float2* __restrict__ coeffs,
                                              const BCase& cb, int M, int R, int s0, int nsig, long cdist, long cstride,
                                              float norm, int lane, int layout)
 {
     if constexpr (NC <= (NN / 2 + 15) / 16) {
         if (nch == NC)
-            fwd_pair<NN, SPW, NC, PREC>(gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
+            fwd_pair<NN, SPW, NC, PREC>(gp, tf, ts, toff, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
                                         lane, layout);
         else
-            fwd_dispatch<NN, SPW, NC + 1, PREC>(nch, gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist,
+            fwd_dispatch<NN, SPW, NC + 1, PREC>(nch, gp, tf, ts, toff, e0, de, coeffs, cb, M, R, s0, nsig, cdist,
                                                 cstride, norm, lane, layout);
     }
 }
@@ -365,8 +389,7 @@ __device__ __forceinline__ void fwd_dispatch(int nch, const float* const (&gp)[S
 // bw.sg (WPB waves x SPW signals).
 template <int NN, int WPB, int SPW, int PREC>
 __device__ __forceinline__ void fwd_item(const PlanTables& t, const long* __restrict__ roff,
-                                         const float* __restrict__ wts32, const float* __restrict__ Tf,
-                                         const SplitTab& ts, const float2* __restrict__ G, float2* __restrict__ coeffs,
+                                         const float* __restrict__ Tf, const SplitTab& ts, const float2* __restrict__ G, float2* __restrict__ coeffs,
                                          int nsig, long cdist, long cstride, float norm, int layout, BlockWork bw)
 {
     constexpr int NKB = NN / 16;
@@ -386,25 +409,23 @@ __device__ __forceinline__ void fwd_item(const PlanTables& t, const long* __rest
         gp[sp] = reinterpret_cast<const float*>(G + (long)min(s0 + sp, nsig - 1) * N3 + cb.vec);
     const long toff = (roff[bw.pi] >> 4) * NKB * 256 + 16 * i + 4 * kk;
     const float* tf = Tf + toff;
-    const float* wl = wts32 + 4 * kk;
     // float offset of this lane's complex pair in beta block 0, and its step per block
     const int p = i & 1;
     const int e0 = 2 * (cb.flip ? NN - 4 - 4 * kk + 2 * (1 - p) : 4 * kk + 2 * p);
     const int de = cb.flip ? -32 : 32;
 
-    fwd_dispatch<NN, SPW, 1, PREC>(nch, gp, tf, ts, toff, wl, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
+    fwd_dispatch<NN, SPW, 1, PREC>(nch, gp, tf, ts, toff, e0, de, coeffs, cb, M, R, s0, nsig, cdist, cstride, norm,
                                    lane, layout);
 }
 
 template <int NN, int WPB, int SPW, int PREC>
 __global__ __launch_bounds__(64 * WPB) void k_dwt_fwd_batch_f32(PlanTables t, const long* __restrict__ roff,
-                                                                const float* __restrict__ wts32,
                                                                 const float* __restrict__ Tf, SplitTab ts,
                                                                 const float2* __restrict__ G, float2* __restrict__ coeffs,
                                                                 int nsig, int nsg, long cdist, long cstride, float norm,
                                                                 int layout)
 {
-    fwd_item<NN, WPB, SPW, PREC>(t, roff, wts32, Tf, ts, G, coeffs, nsig, cdist, cstride, norm, layout,
+    fwd_item<NN, WPB, SPW, PREC>(t, roff, Tf, ts, G, coeffs, nsig, cdist, cstride, norm, layout,
                                  block_work(nsg));
 }
 
@@ -656,7 +677,7 @@ template <bool FWD, int NN>
 __device__ __forceinline__ void multi_item(const BatchMultiGroup& g, BlockWork bw, float* smf, int layout)
 {
     if constexpr (FWD)
-        fwd_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.wts32, g.tab, SplitTab{}, (const float2*)g.in, g.out,
+        fwd_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.tab, SplitTab{}, (const float2*)g.in, g.out,
                                               g.nsig, g.cdist, g.cstride, g.norm, layout, bw);
     else
         inv_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.tab, SplitTab{}, (const float2*)g.in, g.out, g.nsig,
@@ -763,7 +784,7 @@ hipError_t fwd_launch(const PlanTables& t, const BatchTables& bt, const float2* 
     } else {
         const int nsg = (nsig + WPB * SPW - 1) / (WPB * SPW);
         return pfs::launch((k_dwt_fwd_batch_f32<NN, WPB, SPW, PREC>), batch_grid(t.npairs, nsg), dim3(64 * WPB), 0, st,
-                           t, bt.roff, bt.wts32, bt.tf, split_tab(bt, true), G, coeffs, nsig, nsg, cdist, cstride, norm,
+                           t, bt.roff, bt.tf, split_tab(bt, true), G, coeffs, nsig, nsg, cdist, cstride, norm,
                            g_mfma_f32_layout);
     }
 }
@@ -892,7 +913,7 @@ bool batch_supported(int N) { return N == 16 || N == 32 || N == 64 || N == 128 |
 hipError_t launch_wigner_tiles_f32(const PlanTables& t, const BatchTables& bt, hipStream_t st)
 {
     return pfs::launch(k_wigner_tiles_f32, dim3(t.npairs), dim3(t.N < 256 ? t.N : 256), 0, st, t, bt.roff,
-                       (float*)bt.wts32, (float*)bt.tf, (float*)bt.ti);
+                       (float*)bt.tf, (float*)bt.ti);
 }
 
 BatchKernelShape batch_default_shape(int N, bool fwd, int n)

@@ -865,8 +865,7 @@ int ensure_table32(pfs_context* c, Plan& p, hipStream_t st)
     }
     if (rows != batch_table_rows(B)) return fail(PFS_ERR_ARG, "batched table: row count mismatch");
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t o_w = al(pairs.size() * sizeof(long));
-    const size_t o_tf = o_w + al(N * sizeof(float));
+    const size_t o_tf = al(pairs.size() * sizeof(long));
     const size_t tab = (size_t)rows * N * sizeof(float);
     const size_t o_ti = o_tf + al(tab);
     const size_t bytes = o_ti + al(tab);
@@ -875,7 +874,6 @@ int ensure_table32(pfs_context* c, Plan& p, hipStream_t st)
     p.d32_bytes = bytes;
     char* base = (char*)p.d32;
     p.bt.roff = (const long*)base;
-    p.bt.wts32 = (const float*)(base + o_w);
     p.bt.tf = (const float*)(base + o_tf);
     p.bt.ti = (const float*)(base + o_ti);
     HIP_TRY(hipMemcpyAsync((void*)p.bt.roff, roff.data(), pairs.size() * sizeof(long), hipMemcpyHostToDevice, st));
@@ -1366,7 +1364,6 @@ int batch_multi(pfs_context* c, bool fwd, int nsig, const int* bws, const void* 
         BatchMultiGroup& bg = hg[g];
         bg.t = p.t;
         bg.roff = p.bt.roff;
-        bg.wts32 = p.bt.wts32;
         bg.tab = fwd ? p.bt.tf : p.bt.ti;
         bg.in = fwd ? (const void*)r.G : (const void*)r.W;
         bg.out = fwd ? r.W : r.G;

@@ -210,9 +210,8 @@ bool batch_supported(int N);
 // FP32 tables of the batched path for one bandwidth (layout in batch.hip).
 struct BatchTables {
     const long* roff;    // per pair: first table row (multiple of 16)
-    const float* wts32;  // quadrature weights rounded to FP32, N values
-    const float* tf;     // Wigner-d tiles, row-major within a tile
-    const float* ti;     // the same tiles, column-major within a tile
+    const float* tf;     // Wigner-d tiles times the quadrature weights, row-major within a tile
+    const float* ti;     // the Wigner-d tiles (unweighted), column-major within a tile
     // bf16 terms (hi, mid, lo) of tf and ti for the split precisions, same
     // element order; null until a split precision is first used
     const unsigned short* tf16[3];
@@ -256,7 +255,6 @@ hipError_t launch_dwt_inv_batch_f32(const PlanTables& t, const BatchTables& bt, 
 struct BatchMultiGroup {
     PlanTables t;
     const long* roff;
-    const float* wts32;
     const float* tab;
     const void* in;
     float2* out;
