@@ -330,10 +330,13 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
         // a coefficient are adjacent complex values, written by the re lane as
         // one 16-byte store after taking the im values from its partner (a slot
         // past nsig is row padding, never read back)
+        // (four signals per wave: the re lane writes the first two, the im lane
+        // the other two, so every lane stores)
+        constexpr int STEP = SPW % 4 == 0 ? 4 : 2;
 #pragma unroll
-        for (int sp = 0; sp < SPW; sp += 2) {
+        for (int sp = 0; sp < SPW; sp += STEP) {
             if (s0 + sp >= nsig) break;
-            float* cp = reinterpret_cast<float*>(coeffs + (s0 + sp));
+            float* cp = reinterpret_cast<float*>(coeffs + (s0 + sp + (STEP == 4 ? 2 * part : 0)));
 #pragma unroll
             for (int ch = 0; ch < NC; ++ch)
 #pragma unroll
@@ -341,11 +344,17 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
                     const int r = ch * 16 + out_row(lane, v, layout);  // the same in both lanes of a pair
                     if (r < R) {
                         const float sg = (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn);
-                        const float a0 = norm * (sg * acc[ch][sp][v]), a1 = norm * (sg * acc[ch][sp + 1][v]);
-                        const float b0 = swap_pair(a0), b1 = swap_pair(a1);
-                        if (!part)
-                            *reinterpret_cast<float4*>(cp + 2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc)) =
-                                make_float4(a0, b0, a1, b1);
+                        float a[STEP], b[STEP];
+#pragma unroll
+                        for (int q = 0; q < STEP; ++q) {
+                            a[q] = norm * (sg * acc[ch][sp + q][v]);
+                            b[q] = swap_pair(a[q]);
+                        }
+                        float4* dst = reinterpret_cast<float4*>(cp + 2 * cstride * coeff_index32(M + r, cb.Mc, cb.Mpc));
+                        if (STEP == 4)
+                            *dst = part ? make_float4(b[2], a[2], b[3], a[3]) : make_float4(a[0], b[0], a[1], b[1]);
+                        else if (!part)
+                            *dst = make_float4(a[0], b[0], a[1], b[1]);
                     }
                 }
         }
