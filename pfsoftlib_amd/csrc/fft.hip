@@ -417,9 +417,27 @@ template <typename T, int KIND> constexpr size_t r16_lds(int logn)
 {
     return (size_t)r16_lines<T, KIND>() * (16 * ((1 << logn) / 16 + 1) + 1) * (r16_split<T>() ? 1 : 2) * sizeof(T);
 }
+// The complex128 register kernel reads its twiddles from an LDS table the
+// workgroup fills once, instead of 16 global loads per thread held in
+// registers: 158 -> 84 VGPRs, four waves per SIMD (its LDS allows four
+// workgroups per CU).  Measured at B = 128 (interleaved A/B, same results):
+// passes 86.2-90.3 -> 84.6-87.8 us, round trip 549 -> 542.5 us.  The
+// complex64 passes measured the same either way, and the 512-point kernel
+// (160 -> 86-90 VGPRs with the table) slightly slower at B = 256 (round trip
+// 5.51 -> 5.52-5.54 ms): both keep register twiddles.
+template <typename T> constexpr bool r16_lds_twiddles() { return sizeof(T) == 8; }
+template <typename T> constexpr int r16_wpe() { return r16_lds_twiddles<T>() ? 4 : 1; }
+// LDS bytes with the twiddle table (N complex values after the exchange
+// buffer, 16-byte aligned) where the kernel keeps one.
+template <typename T, int KIND> constexpr size_t r16_lds_tw(int logn)
+{
+    return r16_lds_twiddles<T>()
+               ? ((r16_lds<T, KIND>(logn) + 15) & ~(size_t)15) + (size_t)(1 << logn) * 2 * sizeof(T)
+               : r16_lds<T, KIND>(logn);
+}
 
 template <typename T, int LOGN, int KIND, int SIGN>
-__global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
+__global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) __attribute__((amdgpu_waves_per_eu(r16_wpe<T>()))) void k_fft_lines_r16(const typename CT<T>::C* __restrict__ in,
                                                                           typename CT<T>::C* __restrict__ out,
                                                                           LineMap m,
                                                                           const typename CT<T>::C* __restrict__ tw,
@@ -449,10 +467,19 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
 #pragma unroll
         for (int n2 = 0; n2 < 16; ++n2) v[n2] = gload_nt(src, o + n2 * step);
     }
-    // the twiddles of this thread's 16 outputs, in flight with the line loads
-    C w[16];
+    // the twiddles W^(n1 k2) of this thread's 16 outputs: from the workgroup's
+    // LDS table [k2][n1] (consecutive n1 in consecutive slots: no bank
+    // conflicts), or loaded into registers in flight with the line loads
+    C* ltw = reinterpret_cast<C*>(smem_r16 + ((r16_lds<T, KIND>(LOGN) + 15) & ~(size_t)15));
+    C w[r16_lds_twiddles<T>() ? 1 : 16];
+    if constexpr (r16_lds_twiddles<T>()) {
+        for (int e = tid; e < N; e += L * TPL) ltw[e] = gload(tw, (e % TPL) * (e / TPL));
+        __syncthreads();
+    } else {
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) w[k2] = gload(tw, n1 * k2);
+        for (int k2 = 0; k2 < 16; ++k2) w[k2] = gload(tw, n1 * k2);
+    }
+    auto wt = [&](int k2) { return r16_lds_twiddles<T>() ? ltw[k2 * TPL + n1] : w[k2 % (r16_lds_twiddles<T>() ? 1 : 16)]; };
     dft16_perm<SIGN>(v);
 
     // step B: 16 / TPL DFTs of size TPL per thread; row stores with k2 fastest,
@@ -469,7 +496,7 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
         T yi[16];
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
-            C y = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+            C y = twiddle<SIGN>(v[perm_index<16>(k2)], wt(k2));
             ly[k2 * YS + n1] = y.x;
             yi[k2] = y.y;
         }
@@ -490,7 +517,7 @@ __global__ __launch_bounds__((r16_lines<T, KIND>() * (1 << LOGN) / 16)) void k_f
         C* ly = reinterpret_cast<C*>(smem_r16) + la * LS;
         const C* lyb = reinterpret_cast<const C*>(smem_r16) + lb * LS;
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) ly[k2 * YS + n1] = twiddle<SIGN>(v[perm_index<16>(k2)], w[k2]);
+        for (int k2 = 0; k2 < 16; ++k2) ly[k2 * YS + n1] = twiddle<SIGN>(v[perm_index<16>(k2)], wt(k2));
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < Q; ++q)
@@ -764,8 +791,10 @@ static hipError_t launch_pow2(const typename CT<T>::C* in, typename CT<T>::C* ou
                 return pfs::launch((k_fft_lines_r16p<(LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
                                    dim3(L * N / 32), smem, st, in, out, m, tw, scale);
         }
+        const size_t smem_tw = r16_lds_tw<T, KIND>(LOGN);
+        if (smem_tw > lds_limit()) return hipErrorInvalidConfiguration;
         return pfs::launch((k_fft_lines_r16<T, (LOGN >= 6 && LOGN <= 8 ? LOGN : 8), KIND, SIGN>), grid,
-                           dim3(L * N / 16), smem, st, in, out, m, tw, scale);
+                           dim3(L * N / 16), smem_tw, st, in, out, m, tw, scale);
     }
     constexpr int LINES = N >= 8 ? (N >= 1024 ? 2 : (N >= 512 ? 4 : 8)) : N;
     size_t smem = (size_t)LINES * (N + 1) * sizeof(typename CT<T>::C);
