@@ -325,14 +325,15 @@ __device__ __forceinline__ void fwd_pair(const float* const (&gp)[SPW], const fl
 
     if (!cb.active) return;  // lanes 2c, 2c+1 (case c, re / im) leave together
     const int part = lane & 1;
-    if (SPW % 2 == 0 && cdist == 1 && (cstride & 1) == 0) {
-        // interleaved layout, even row length: signals s0 + sp, s0 + sp + 1 of
-        // a coefficient are adjacent complex values, written by the re lane as
-        // one 16-byte store after taking the im values from its partner (a slot
-        // past nsig is row padding, never read back)
-        // (four signals per wave: the re lane writes the first two, the im lane
-        // the other two, so every lane stores)
-        constexpr int STEP = SPW % 4 == 0 ? 4 : 2;
+    constexpr int STEP = SPW % 4 == 0 ? 4 : 2;
+    if (SPW % 2 == 0 && cdist == 1 && cstride % STEP == 0) {
+        // interleaved layout, row length a multiple of STEP: signals s0 + sp,
+        // s0 + sp + 1 of a coefficient are adjacent complex values, written by
+        // the re lane as one 16-byte store after taking the im values from its
+        // partner (a slot past nsig lies in the row's padding, since s0 + sp
+        // < nsig <= cstride and both are multiples of STEP, and is never read
+        // back).  With four signals per wave the re lane writes the first two,
+        // the im lane the other two, so every lane stores.
 #pragma unroll
         for (int sp = 0; sp < SPW; sp += STEP) {
             if (s0 + sp >= nsig) break;
