@@ -94,6 +94,39 @@ __global__ void k_plan_powers(int B, const double* sinh, const double* cosh, dou
     if (e < 2 * B - 1) pwc[(long)e * N + k] = ipow(cosh[k], e);
 }
 
+// The rescaled recurrence of each pair (M, M'), one thread per pair walking
+// its rows in order.  With d_r = P_r e_r, P_0 = P_1 = 1 and P_{r+1} =
+// -c1_r P_{r-1} (c1_r < 0 for 1 <= r < R - 1; c1_0 = 0 and the row before 0
+// is zero), the step d_{r+1} = (f1 cb + f1f2) d_r + c1 d_{r-1} becomes
+// e_{r+1} = (a cb + b) e_r - e_{r-1} with (a, b) = (f1, f1f2) P_r / P_{r+1}:
+// one fused multiply-add per node and degree fewer.  P stays within
+// [0.11, 1.13] for every pair up to B = 256.  Rows from B - M - 1 on have
+// a = b = 0; P is zero from B - M on (the pair's padding rows).
+__global__ void k_plan_rescale(int B, const long* rsm, double2* rab, double* rp)
+{
+    const int M = blockIdx.x;
+    for (int Mp = threadIdx.x; Mp <= M; Mp += blockDim.x) {
+        const int R = B - M, RP = ((R + 15) >> 4) << 4;
+        const long o = rsm[M] + (long)Mp * RP;
+        double pm = 1.0, pc = 1.0;  // P_{r-1}, P_r
+        for (int r = 0; r < RP; ++r) {
+            double f1, f1f2, c1;
+            recurrence_step(M, Mp, r, R, f1, f1f2, c1);
+            if (r < R - 1) {
+                const double pn = r == 0 ? 1.0 : -c1 * pm;
+                const double rho = pc / pn;
+                rab[o + r] = make_double2(f1 * rho, f1f2 * rho);
+                rp[o + r] = pc;
+                pm = pc;
+                pc = pn;
+            } else {
+                rab[o + r] = make_double2(0.0, 0.0);
+                rp[o + r] = r < R ? pc : 0.0;
+            }
+        }
+    }
+}
+
 __global__ void k_plan_normfac(int B, double* nf)
 {
     int M = blockIdx.x, Mp = threadIdx.x;
@@ -474,7 +507,20 @@ int get_plan(pfs_context* c, int B, Plan** out)
     size_t off_tw32 = off_tw + align(N * sizeof(double2));
     size_t off_pws = off_tw32 + align(N * sizeof(float2));
     size_t off_pwc = off_pws + align((size_t)B * N * sizeof(double));
-    size_t total = off_pwc + align((size_t)(2 * B - 1) * N * sizeof(double));
+    // rescaled-recurrence rows (16 ceil((B - M) / 16) per pair, pairs in
+    // (M, M') order), read by the inverse DWT at 2B = 512 only (67 MB at
+    // B = 256)
+    const bool rescaled = N > 256;
+    std::vector<long> rsm(B, 0);
+    size_t nrows = 0;
+    for (int M = 0; M < B && rescaled; ++M) {
+        rsm[M] = (long)nrows;
+        nrows += (size_t)(M + 1) * (((B - M + 15) >> 4) << 4);
+    }
+    size_t off_rsm = off_pwc + align((size_t)(2 * B - 1) * N * sizeof(double));
+    size_t off_rab = off_rsm + align(rsm.size() * sizeof(long));
+    size_t off_rp = off_rab + align(nrows * sizeof(double2));
+    size_t total = off_rp + align(nrows * sizeof(double));
 
     Plan p;
     p.B = B;
@@ -498,6 +544,9 @@ int get_plan(pfs_context* c, int B, Plan** out)
     p.t.tw32 = (float2*)(base + off_tw32);
     p.t.pws = (double*)(base + off_pws);
     p.t.pwc = (double*)(base + off_pwc);
+    p.t.rsm = rescaled ? (const long*)(base + off_rsm) : nullptr;
+    p.t.rab = rescaled ? (const double2*)(base + off_rab) : nullptr;
+    p.t.rp = rescaled ? (const double*)(base + off_rp) : nullptr;
     hipError_t e = hipMemcpyAsync((void*)p.t.pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
                                   c->stream);
     if (e == hipSuccess)
@@ -513,6 +562,11 @@ int get_plan(pfs_context* c, int B, Plan** out)
     if (e == hipSuccess)
         e = pfs::launch(k_plan_powers, dim3((N + 127) / 128, 2 * B - 1), dim3(128), 0, c->stream, B, p.t.sinh, p.t.cosh,
                         (double*)p.t.pws, (double*)p.t.pwc);
+    if (e == hipSuccess && rescaled)
+        e = hipMemcpyAsync((void*)p.t.rsm, rsm.data(), rsm.size() * sizeof(long), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && rescaled)
+        e = pfs::launch(k_plan_rescale, dim3(B), dim3(64), 0, c->stream, B, p.t.rsm, (double2*)p.t.rab,
+                        (double*)p.t.rp);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         ctx_free(c, p.mem, p.bytes);

@@ -73,29 +73,6 @@ __device__ __forceinline__ CaseInfo case_info(int c, int M, int Mp, int N)
     return ci;
 }
 
-// Recurrence coefficients for the step from degree l = M + r to l + 1
-// (fn_dwt.hpp:167-192), FP64: (f1, f1*f2, c1), zero from the last row on so
-// the padded rows stay 0.
-__device__ __forceinline__ void recurrence_step(int M, int Mp, int r, int R, double& f1, double& f1f2, double& c1)
-{
-    f1 = 0, f1f2 = 0, c1 = 0;
-    if (r < R - 1) {
-        double idx = M + r;
-        double nrm = sqrt((2.0 * idx + 3.0) / (2.0 * idx + 1.0));
-        double nom = (idx + 1.0) * (2.0 * idx + 1.0);
-        double den = 1.0 / sqrt(((idx + 1) * (idx + 1) - (double)M * M) * ((idx + 1) * (idx + 1) - (double)Mp * Mp));
-        f1 = nrm * nom * den;
-        double f2 = 0;
-        if (M + r != 0) {
-            double t1 = sqrt((2.0 * idx + 3.0) / (2.0 * idx - 1.0)) * (idx + 1.0) / idx;
-            double t2 = sqrt((idx * idx - (double)M * M) * (idx * idx - (double)Mp * Mp));
-            c1 = -t1 * t2 * den;
-            f2 = -(double)(M * Mp) / (idx * (idx + 1.0));
-        }
-        f1f2 = f1 * f2;
-    }
-}
-
 // The same, stored as three rows of `stride` values.
 __device__ __forceinline__ void recurrence_coeffs(int M, int Mp, int r, int R, double* cf, int stride)
 {
@@ -290,6 +267,19 @@ __device__ __forceinline__ void recurrence_coeffs4(int M, int Mp, int R, int RP,
     for (int r = first; r < RP; r += stride) {
         recurrence_step(M, Mp, r, R, cf4[4 * r], cf4[4 * r + 1], cf4[4 * r + 2]);
         cf4[4 * r + 3] = 0.0;
+    }
+}
+
+// The rescaled recurrence rows of pair (M, M') interleaved per degree:
+// cf4[4 r] = (a_r, b_r, P_r, 0), r = first, first + stride, ... < RP.
+__device__ __forceinline__ void rescaled_coeffs4(const PlanTables& t, int M, int Mp, int RP, double* cf4, int first,
+                                                 int stride)
+{
+    const long o = t.rsm[M] + (long)Mp * RP;
+    for (int r = first; r < RP; r += stride) {
+        const double2 ab = t.rab[o + r];
+        *reinterpret_cast<double2*>(cf4 + 4 * r) = ab;
+        *reinterpret_cast<double2*>(cf4 + 4 * r + 2) = make_double2(t.rp[o + r], 0.0);
     }
 }
 
@@ -948,7 +938,14 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_valu(PlanTables t, con
 //   - full chunks run unrolled with no bounds test; the last R % 16 degrees
 //     of a pair run one at a time (no padded degree is computed).
 // Results equal k_dwt_inv_f64_valu's: the same operations in the same order.
-template <int NCH, int WPP>
+// With RESC the chains run the plan's rescaled recurrence instead (e_{r+1} =
+// (a_r cb + b_r) e_r - e_{r-1}, d_r = P_r e_r, the cf4 rows (a, b, P, 0)):
+// two FMAs per chain and degree where the reference's form takes three, P_r
+// folded into the staged coefficients, and results equal to rounding.
+// Measured (interleaved A/B, same rounding-level results): 2B = 512 inverse
+// 1.092 -> 1.063 ms; 2B = 256 92.4 -> 92.9 us (the table costs more than the
+// instruction saves there), so only 2B = 512 runs it.
+template <int NCH, int WPP, bool RESC>
 __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, const double2* __restrict__ coeffs,
                                                                  double2* __restrict__ G, double norm,
                                                                  const double2* __restrict__ nq_grid,
@@ -992,7 +989,8 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
 #pragma unroll
     for (int i = 0; i < 4; ++i) nxt[i] = csrc[cidx(dl + 4 * i)];
 
-    recurrence_coeffs4(M, Mp, R, RP, cf4, tid, 64 * WPP);
+    if constexpr (RESC) rescaled_coeffs4(t, M, Mp, RP, cf4, tid, 64 * WPP);
+    else recurrence_coeffs4(M, Mp, R, RP, cf4, tid, 64 * WPP);
     Chains<NCH> ch;
     init_chains<NCH>(ch, ct, nf, kw + lane, N);
 
@@ -1003,9 +1001,10 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
         for (int q = 0; q < 16; ++q) acc[c][q] = 0.0;
     __syncthreads();  // the step-coefficient table is complete
 
-    // one degree: acc[c][q] += c_q * d(c) over the 16 right-hand sides, then
-    // the chains step to the next degree
-    auto degree = [&](double cv, double f1, double f1f2, double c1) {
+    // one degree: acc[c][q] += c_q * d(c) (RESC: (P_r c_q) * e(c)) over the
+    // 16 right-hand sides, then the chains step to the next degree with the
+    // step coefficients (x, y, z) = (f1, f1 f2, c1) (RESC: (a, b, -))
+    auto degree = [&](double cv, double x, double y, double z) {
 #pragma unroll
         for (int q = 0; q < 16; ++q)
 #pragma unroll
@@ -1015,19 +1014,20 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
             }
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            const double u = fma(f1, ch.cb[c], f1f2);
-            const double dn = fma(ch.cur[c], u, c1 * ch.prev[c]);
+            const double u = fma(x, ch.cb[c], y);
+            const double dn = RESC ? fma(ch.cur[c], u, -ch.prev[c]) : fma(ch.cur[c], u, z * ch.prev[c]);
             ch.prev[c] = ch.cur[c];
             ch.cur[c] = dn;
         }
     };
-    // stage a chunk's values (loaded a chunk ago); the wave's LDS operations
-    // run in order, so the writes land after the previous chunk's reads and
-    // before this chunk's
-    auto stage = [&]() {
+    // stage a chunk's values (loaded a chunk ago; RESC: each times its
+    // degree's P_r); the wave's LDS operations run in order, so the writes
+    // land after the previous chunk's reads and before this chunk's
+    auto stage = [&](int chk) {
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) stg[lane + 64 * i] = nxt[i] * fac;
+        for (int i = 0; i < 4; ++i)
+            stg[lane + 64 * i] = RESC ? nxt[i] * (fac * cf4[4 * (16 * chk + dl + 4 * i) + 2]) : nxt[i] * fac;
         __builtin_amdgcn_wave_barrier();
     };
 
@@ -1035,32 +1035,33 @@ __global__ __launch_bounds__(64 * WPP) void k_dwt_inv_f64_staged(PlanTables t, c
     // degree ahead
     const int nfull = R >> 4, rem = R & 15;
     for (int chk = 0; chk < nfull; ++chk) {
-        stage();
+        stage(chk);
         const int rn = (chk + 1) * 16;
 #pragma unroll
         for (int i = 0; i < 4; ++i) nxt[i] = csrc[cidx(rn + dl + 4 * i)];
         const double* cfc = cf4 + 4 * 16 * chk;
-        double cvn = stg[j], c1n = cfc[2];
+        double cvn = stg[j], c1n = RESC ? 0.0 : cfc[2];
         double2 s01n = *reinterpret_cast<const double2*>(cfc);
 #pragma unroll
         for (int rr = 0; rr < 16; ++rr) {
-            const double cv = cvn, f1 = s01n.x, f1f2 = s01n.y, c1 = c1n;
+            const double cv = cvn, c1 = c1n;
+            const double2 s01 = s01n;
             if (rr < 15) {
                 cvn = stg[16 * (rr + 1) + j];
                 s01n = *reinterpret_cast<const double2*>(cfc + 4 * (rr + 1));
-                c1n = cfc[4 * (rr + 1) + 2];
+                if (!RESC) c1n = cfc[4 * (rr + 1) + 2];
             }
-            degree(cv, f1, f1f2, c1);
+            degree(cv, s01.x, s01.y, c1);
         }
     }
     // the last rem < 16 degrees one at a time (no padded degrees run)
     if (rem) {
-        stage();
+        stage(nfull);
         const double* cfc = cf4 + 4 * 16 * nfull;
 #pragma nounroll
         for (int rr = 0; rr < rem; ++rr) {
             const double2 s01 = *reinterpret_cast<const double2*>(cfc + 4 * rr);
-            degree(stg[16 * rr + j], s01.x, s01.y, cfc[4 * rr + 2]);
+            degree(stg[16 * rr + j], s01.x, s01.y, RESC ? 0.0 : cfc[4 * rr + 2]);
         }
     }
 
@@ -1198,10 +1199,16 @@ hipError_t launch_dwt_inv_f64(const PlanTables& t, const double2* coeffs, double
         if (staged) {
             if (qs != 1) return hipErrorInvalidValue;
             const size_t ssmem = (4 * (size_t)(((t.B + 15) / 16) * 16) + 256 * (size_t)wpp) * sizeof(double);
+            // 2B = 512 (four chains x two waves): the rescaled recurrence
+            if (t.N > 256)
+                return nch == 4 && wpp == 2 && t.rab
+                           ? pfs::launch((k_dwt_inv_f64_staged<4, 2, true>), grid, block, ssmem, st, t, coeffs, G,
+                                         norm, nq_grid, nq_compact)
+                           : hipErrorInvalidValue;
 #define PFS_INVS(NC, W)                                                                                     \
             if (nch == NC && wpp == W)                                                                      \
-                return pfs::launch((k_dwt_inv_f64_staged<NC, W>), grid, block, ssmem, st, t, coeffs, G, norm, nq_grid, \
-                                   nq_compact);
+                return pfs::launch((k_dwt_inv_f64_staged<NC, W, false>), grid, block, ssmem, st, t, coeffs, G, norm, \
+                                   nq_grid, nq_compact);
             PFS_INVS(1, 1)
             PFS_INVS(1, 2)
             PFS_INVS(2, 2)

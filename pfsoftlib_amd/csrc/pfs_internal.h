@@ -120,6 +120,29 @@ __host__ __device__ __forceinline__ double ipow(double x, int e)
     return r;
 }
 
+// Recurrence coefficients for the step from degree l = M + r to l + 1
+// (fn_dwt.hpp:167-192), FP64: (f1, f1*f2, c1), zero from the last row on so
+// the padded rows stay 0.
+__host__ __device__ __forceinline__ void recurrence_step(int M, int Mp, int r, int R, double& f1, double& f1f2, double& c1)
+{
+    f1 = 0, f1f2 = 0, c1 = 0;
+    if (r < R - 1) {
+        double idx = M + r;
+        double nrm = sqrt((2.0 * idx + 3.0) / (2.0 * idx + 1.0));
+        double nom = (idx + 1.0) * (2.0 * idx + 1.0);
+        double den = 1.0 / sqrt(((idx + 1) * (idx + 1) - (double)M * M) * ((idx + 1) * (idx + 1) - (double)Mp * Mp));
+        f1 = nrm * nom * den;
+        double f2 = 0;
+        if (M + r != 0) {
+            double t1 = sqrt((2.0 * idx + 3.0) / (2.0 * idx - 1.0)) * (idx + 1.0) / idx;
+            double t2 = sqrt((idx * idx - (double)M * M) * (idx * idx - (double)Mp * Mp));
+            c1 = -t1 * t2 * den;
+            f2 = -(double)(M * Mp) / (idx * (idx + 1.0));
+        }
+        f1f2 = f1 * f2;
+    }
+}
+
 // Device-resident tables for one bandwidth (all FP64 unless noted).
 struct PlanTables {
     int B, N;
@@ -131,6 +154,13 @@ struct PlanTables {
     const double* pws;     // sin(beta_k / 2)^e, [e * N + k], e < B (ipow)
     const double* pwc;     // cos(beta_k / 2)^e, [e * N + k], e < 2B - 1 (ipow)
     const double* normfac; // sqrt((2M+1)/2) * sqrt(C(2M, M-M')), [M*B + M'], M >= M' >= 0
+    // The rescaled recurrence of pair (M, M'), rows r < RP = 16 ceil((B-M)/16)
+    // at rsm[M] + M' RP + r: e_{r+1} = (a_r cos(beta) + b_r) e_r - e_{r-1}
+    // with rab[.] = (a_r, b_r), and d^{M+r} = rp[.] e_r (zero rows from B - M
+    // on).  Made for 2B = 512 plans only (null otherwise).
+    const long* rsm;
+    const double2* rab;
+    const double* rp;
     const int2* pairs;     // canonical (M, M'), M >= M' >= 0, heaviest first
     const int2* xpairs;    // the same pairs in the FP64 DWT launch order (workgroup b runs xpairs[b])
     const double2* tw;     // exp(-2 pi i j / N), j < N

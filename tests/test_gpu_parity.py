@@ -209,11 +209,13 @@ def test_inverse_dwt_kernels_match_oracle(ctx, torch, b128, kind):
 
 
 @pytest.mark.parametrize("pair", [(5, 13), (7, 15)])
-def test_staged_inverse_bit_identical(ctx, torch, pair):
+def test_staged_inverse_matches_valu(ctx, torch, pair):
     # the staged inverse runs the vector-ALU kernel's operations in the same
     # order, so the grids agree exactly (zeros up to their sign), at B = 128,
-    # 2B = 512 (four chains x two waves for both), sizes that leave chains or
-    # chunk rows empty, and through the device route with stale Nyquist entries
+    # sizes that leave chains or chunk rows empty, and through the device
+    # route with stale Nyquist entries; at 2B = 512 (four chains x two waves
+    # for both) it runs the rescaled recurrence (d_r = P_r e_r) and agrees to
+    # rounding
     base, staged = pair
     default = ctx.tune_get("dwt_inv_kernel")
     try:
@@ -225,7 +227,10 @@ def test_staged_inverse_bit_identical(ctx, torch, pair):
             for kind in (base, staged):
                 ctx.tune("dwt_inv_kernel", kind)
                 out.append(ctx.idsoft(B, cs))
-            assert np.array_equal(out[0], out[1]), B
+            if B > 128:
+                assert rel(out[0], out[1]) < 1e-13, B
+            else:
+                assert np.array_equal(out[0], out[1]), B
         Bs = 64
         N = 2 * Bs
         cs = torch.from_numpy(O.seeded_coeffs(Bs, 8)).cuda()
