@@ -192,10 +192,13 @@ def cpu_baseline(B, pairs, seed, sweep_B=64):
 
     t_inv, t_fwd, r = run(pairs, threads)
     err = float(np.linalg.norm(r - c) / np.linalg.norm(c))
+    progress("CPU baseline: %d round trips on %d threads done" % (pairs, threads))
     s_inv, s_fwd, _ = run(1, 1)
+    progress("CPU baseline: serial round trip done")
     par = (sum(t_inv) + sum(t_fwd)) / pairs
     ser = s_inv[0] + s_fwd[0]
     sweep = thread_sweep(O, sweep_B, threads, seed) if sweep_B else None
+    progress("CPU baseline: thread sweep done")
     fft = fft_share(O, np, B, O.idsoft(B, c, threads=threads), threads, par)
     return {
         "value": 1.0 / par,
@@ -750,6 +753,11 @@ def copy_calibration(torch, P, steps, mib):
     ctx.close()
 
 
+def progress(msg):
+    """One line on stderr as each leg of the run ends (stdout keeps only the JSON line)."""
+    print("bench.py: " + msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -872,6 +880,7 @@ def main():
         step()
     torch.cuda.synchronize()
     ctx.timing(False)
+    progress("warm-up and kernel breakdown done (B = %d)" % B)
     kernels = kernel_table(ctx.timing_collect(), nbreak)
     dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
 
@@ -977,12 +986,14 @@ def main():
         g_host = ctx.idsoft(B, c_host)
         ctx.dsoft(B, g_host)
     out["host_buffer_pairs_per_s"] = 3 / (time.perf_counter() - t2)
+    progress("timed region done: %.1f pairs/s; host-buffer leg done" % (args.steps / elapsed))
     del grid, back
     if args.batch_signals and B <= 128:  # the batched path serves B <= 128
         bl = batched_leg(torch, P, ctx, B, args.batch_signals, args.batch_steps, 1, args.seed, args.batch_check)
         out["batched_mixed"] = bl
         out["batched_vs_plain_fp64"] = bl["pairs_per_s"] / out["value"]
         torch.cuda.empty_cache()
+        progress("batched leg done")
     if not args.no_c4:
         out["batched_mixed_512xB64"] = batched_leg(torch, P, ctx, 64, 512, args.batch_steps, 1, args.seed, args.batch_check)
         # the split-bf16 contractions on the same workload, with their error against FP64
@@ -992,13 +1003,16 @@ def main():
                 k: r[k] for k in ("precision", "pairs_per_s", "ms_per_batch_roundtrip", "rel_l2_vs_fp64")}
             out["batched_512xB64_bf16x%d" % prec]["kernels"] = {
                 k: v for k, v in r["kernels"].items() if k.startswith("batch_dwt_")}
+        progress("512 x B = 64 legs done")
     if args.serve_seconds > 0:
         out["serving"] = serving_leg(torch, P, ctx, args.serve_seconds, args.seed, int(args.serve_budget_gb * 2 ** 30),
                                      args.serve_slots, args.serve_max_batch, rate=args.serve_rate or None)
         torch.cuda.empty_cache()
         out["serving_multi_batch"] = multi_batch_leg(torch, P, ctx, args.serve_seconds, args.seed,
                                                      out["serving"]["offered_rate_per_s"])
+        progress("serving legs done")
     out["dwt_accuracy"] = dwt_accuracy(ctx, np, 256, 10, args.seed)
+    progress("DWT accuracy table done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(B, args.cpu_pairs, args.seed)
         out["cpu_other_configs"] = cpu_other_configs(args.seed)
