@@ -563,17 +563,29 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
 // gathered per lane.  The 8 beta-vectors of each signal are staged per wave
 // in LDS and written out as contiguous pencils.
 // One work item of the inverse contraction; smf: WPB x NN x OSTRF floats of LDS.
-template <int NN, int WPB, int SPW, int PREC, bool PAIRED>
+// Workgroup-staged coefficients (GATHER 2): the WPB x SPW = SG signals of the
+// workgroup are adjacent in each interleaved coefficient row, so one chunk's
+// operands (16 degrees x 8 cases x SG signals) are SG / 2 16-byte loads per
+// (degree, case), spread over the workgroup's threads, and reach the lanes
+// through LDS as [degree][case][re / im][signal]: one coefficient index per
+// thread and chunk, and each cache line read once per workgroup instead of
+// once per wave.  Two chunk buffers (CF floats each) alias the output
+// staging, which is not in use until the last chunk.
+template <int NN, int WPB, int SPW, int PREC, int GATHER>
 __device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* __restrict__ roff,
                                               const float* __restrict__ Ti, const SplitTab& ts,
                                               const float2* __restrict__ coeffs, float2* __restrict__ G, int nsig,
-                                              long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf)
+                                              long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf,
+                                              int wreg)
 {
+    constexpr bool PAIRED = GATHER == 1;
     constexpr int NKB = NN / 16;
+    constexpr int SG = WPB * SPW;
     if (bw.pi >= t.npairs) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s0 = (bw.sg * WPB + w) * SPW;
-    if (s0 >= nsig) return;
+    // with staged coefficients every wave takes part in the chunk barriers
+    if (GATHER != 2 && s0 >= nsig) return;
     const int2 pr = t.pairs[bw.pi];
     const int M = pr.x, Mp = pr.y, R = t.B - M, nch = (R + 15) >> 4;
     const int i = lane & 15, kk = lane >> 4, part = i & 1;
@@ -604,18 +616,86 @@ __device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* _
     InvTab<NKB, PREC> tab;
 #pragma unroll
     for (int kb = 0; kb < NKB; ++kb) inv_tab_load<NKB, PREC>(tab, kb, ti, ts, toff);
-    inv_gather<SPW, PAIRED>(bc, cs, cp, cb, M, R, cstride, norm, kk, 0, part);
-    for (int q = 0; q < nch; ++q) {
-        const int qn = q + 1 < nch ? q + 1 : q;
-        inv_gather<SPW, PAIRED>(bn, cs, cp, cb, M, R, cstride, norm, kk, qn, part);
-        __builtin_amdgcn_sched_barrier(0);
-        inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
-        __builtin_amdgcn_sched_barrier(0);
-        bc = bn;
+    if constexpr (GATHER == 2) {
+        // loader role: items e = tid + 64 WPB j (j < SPW) of the chunk's
+        // 16 x 8 x SG / 2 16-byte loads: (degree rl, case cl, signal pair h)
+        constexpr int CF = 16 * 8 * 2 * SG;
+        const int tid = threadIdx.x, sbase = bw.sg * SG;
+        int lrl[SPW], lh[SPW];
+        BCase lcb[SPW];
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) {
+            const int e = tid + 64 * WPB * j;
+            lh[j] = e % (SG / 2);
+            const int rc = e / (SG / 2);
+            lrl[j] = rc >> 3;
+            lcb[j] = bcase(rc & 7, M, Mp, NN);
+        }
+        const float* cbase = reinterpret_cast<const float*>(coeffs) + 2 * sbase;
+        float4 ld[SPW];
+        auto load = [&](int q) {
+#pragma unroll
+            for (int j = 0; j < SPW; ++j) {
+                const int r = 16 * q + lrl[j];
+                const long ci = (lcb[j].active && r < R) ? 2 * cstride * coeff_index32(M + r, lcb[j].Mc, lcb[j].Mpc) : 0;
+                ld[j] = *reinterpret_cast<const float4*>(cbase + ci + 4 * lh[j]);
+            }
+        };
+        auto put = [&](int buf) {
+            float* st = smf + buf * CF;
+#pragma unroll
+            for (int j = 0; j < SPW; ++j) {
+                const int e = tid + 64 * WPB * j;
+                const int rc = e / (SG / 2);  // rl * 8 + case
+                *reinterpret_cast<float2*>(st + (2 * rc) * SG + 2 * lh[j]) = make_float2(ld[j].x, ld[j].z);
+                *reinterpret_cast<float2*>(st + (2 * rc + 1) * SG + 2 * lh[j]) = make_float2(ld[j].y, ld[j].w);
+            }
+        };
+        // this lane's operands of chunk q: degrees 16 q + 4 kk + u, its case
+        // and part, signals w SPW .. w SPW + SPW - 1
+        auto take = [&](InvStage<SPW>& S, int buf, int q) {
+            const float* st = smf + buf * CF;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = 16 * q + 4 * kk + u;
+                const bool live = cb.active && r < R;
+                S.f[u] = live ? norm * (float)((cb.flip && (r & 1)) ? -cb.sgn : cb.sgn) : 0.0f;
+                const float* src = st + (2 * ((4 * kk + u) * 8 + (i >> 1)) + part) * SG + w * SPW;
+#pragma unroll
+                for (int sp = 0; sp < SPW; ++sp) S.v[sp][u] = src[sp];
+            }
+        };
+        load(0);
+        // a workgroup's waves may still be reading a previous item's output
+        // staging, which the chunk buffers alias
+        __syncthreads();
+        put(0);
+        for (int q = 0; q < nch; ++q) {
+            if (q + 1 < nch) load(q + 1);
+            __syncthreads();  // chunk q staged by every thread; chunk q - 1's reads done
+            take(bc, q & 1, q);
+            const int qn = q + 1 < nch ? q + 1 : q;
+            inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
+            if (q + 1 < nch) put((q + 1) & 1);
+        }
+        __syncthreads();  // the last chunk's reads done before the output staging reuses the region
+        if (s0 >= nsig) return;
+    } else {
+        inv_gather<SPW, PAIRED>(bc, cs, cp, cb, M, R, cstride, norm, kk, 0, part);
+        for (int q = 0; q < nch; ++q) {
+            const int qn = q + 1 < nch ? q + 1 : q;
+            inv_gather<SPW, PAIRED>(bn, cs, cp, cb, M, R, cstride, norm, kk, qn, part);
+            __builtin_amdgcn_sched_barrier(0);
+            inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
+            __builtin_amdgcn_sched_barrier(0);
+            bc = bn;
+        }
     }
 
     // write each signal's 8 beta-vectors, staged so every case run is contiguous
-    float* sw = smf + (long)w * NN * OSTRF;
+    // wave w's region: wreg floats (>= NN OSTRF; the largest NN of a
+    // persistent launch, so items of different sizes never share a region)
+    float* sw = smf + (long)w * wreg;
 #pragma unroll
     for (int sp = 0; sp < SPW; ++sp) {
         const int s = s0 + sp;
@@ -648,20 +728,35 @@ __device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* _
 // signals (interleaved A/B, two runs each, same results): 2B = 128 inverse
 // 3.30 -> 3.01-3.03 ms (512 x B = 64: 25.0 k -> 25.4 k pairs/s); 2B = 256
 // 3.61 -> 3.66-3.67 ms, so 2B = 256 keeps the 8-byte gathers.
-template <int NN, int WPB, int SPW, int PREC>
+template <int NN, int WPB, int SPW, int PREC, bool LDSC = false>
 __device__ __forceinline__ void inv_item(const PlanTables& t, const long* __restrict__ roff,
                                          const float* __restrict__ Ti, const SplitTab& ts,
                                          const float2* __restrict__ coeffs, float2* __restrict__ G, int nsig,
-                                         long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf)
+                                         long cdist, long cstride, float norm, int layout, BlockWork bw, float* smf,
+                                         int wreg = NN * OSTRF)
 {
-    if constexpr (SPW % 2 == 0 && NN <= 128) {
-        if (cdist == 1 && (cstride & 1) == 0) {
-            inv_item_body<NN, WPB, SPW, PREC, true>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw,
-                                                    smf);
+    // workgroup-staged coefficients (LDSC: the single-bandwidth kernel): the
+    // two chunk buffers fit in the output staging (2 x 256 SG <= WPB NN OSTRF
+    // floats for NN >= 128), every signal group inside the padded row.
+    // Measured (interleaved A/B, two runs each, same results): 64 signals at
+    // 2B = 256 3.52-3.53 -> 3.34-3.36 ms, 512 signals at 2B = 128 2.91 ->
+    // 2.74 ms, against the gathers below
+    if constexpr (LDSC && NN >= 128 && (WPB * SPW) % 2 == 0) {
+        if (cdist == 1 && cstride % (WPB * SPW) == 0) {
+            inv_item_body<NN, WPB, SPW, PREC, 2>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw,
+                                                 smf, wreg);
             return;
         }
     }
-    inv_item_body<NN, WPB, SPW, PREC, false>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw, smf);
+    if constexpr (SPW % 2 == 0 && NN <= 128) {
+        if (cdist == 1 && (cstride & 1) == 0) {
+            inv_item_body<NN, WPB, SPW, PREC, 1>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw,
+                                                 smf, wreg);
+            return;
+        }
+    }
+    inv_item_body<NN, WPB, SPW, PREC, 0>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, bw, smf,
+                                         wreg);
 }
 
 template <int NN, int WPB, int SPW, int PREC>
@@ -672,7 +767,8 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
                                                                 int layout)
 {
     extern __shared__ __align__(16) float smf[];
-    inv_item<NN, WPB, SPW, PREC>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, block_work(nsg), smf);
+    inv_item<NN, WPB, SPW, PREC, true>(t, roff, Ti, ts, coeffs, G, nsig, cdist, cstride, norm, layout, block_work(nsg),
+                                       smf);
 }
 
 // ---- contraction over a batch of mixed bandwidths -----------------------------
@@ -684,14 +780,18 @@ __global__ __launch_bounds__(64 * WPB) void k_dwt_inv_batch_f32(PlanTables t, co
 // round-robin dispatch) they share one XCD's L2 copy of the pair's tiles.
 // Each item runs the single-bandwidth contraction body of its group's 2B.
 template <bool FWD, int NN>
-__device__ __forceinline__ void multi_item(const BatchMultiGroup& g, BlockWork bw, float* smf, int layout)
+__device__ __forceinline__ void multi_item(const BatchMultiGroup& g, BlockWork bw, float* smf, int layout, int wreg)
 {
     if constexpr (FWD)
         fwd_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.tab, SplitTab{}, (const float2*)g.in, g.out,
                                               g.nsig, g.cdist, g.cstride, g.norm, layout, bw);
     else
-        inv_item<NN, MULTI_WPB, MULTI_SPW, 0>(g.t, g.roff, g.tab, SplitTab{}, (const float2*)g.in, g.out, g.nsig,
-                                              g.cdist, g.cstride, g.norm, layout, bw, smf);
+        // per-lane gathers: the staged coefficients' per-chunk barriers
+        // measured slower in this persistent kernel (fixed 64-request batch
+        // of the serving mix 1.545 -> 1.636 ms), whose waves otherwise drift
+        // apart across items
+        inv_item<NN, MULTI_WPB, MULTI_SPW, 0, false>(g.t, g.roff, g.tab, SplitTab{}, (const float2*)g.in, g.out, g.nsig,
+                                              g.cdist, g.cstride, g.norm, layout, bw, smf, wreg);
 }
 
 template <bool FWD, int NNMAX>
@@ -706,11 +806,11 @@ __global__ __launch_bounds__(64 * MULTI_WPB) void k_dwt_batch_multi(const BatchM
         const BatchMultiGroup& g = groups[w.x];
         const BlockWork bw{w.y, w.z};
         switch (g.t.N) {
-        case 16: multi_item<FWD, 16>(g, bw, smf, layout); break;
-        case 32: if constexpr (NNMAX >= 32) multi_item<FWD, 32>(g, bw, smf, layout); break;
-        case 64: if constexpr (NNMAX >= 64) multi_item<FWD, 64>(g, bw, smf, layout); break;
-        case 128: if constexpr (NNMAX >= 128) multi_item<FWD, 128>(g, bw, smf, layout); break;
-        default: if constexpr (NNMAX >= 256) multi_item<FWD, 256>(g, bw, smf, layout); break;
+        case 16: multi_item<FWD, 16>(g, bw, smf, layout, NNMAX * OSTRF); break;
+        case 32: if constexpr (NNMAX >= 32) multi_item<FWD, 32>(g, bw, smf, layout, NNMAX * OSTRF); break;
+        case 64: if constexpr (NNMAX >= 64) multi_item<FWD, 64>(g, bw, smf, layout, NNMAX * OSTRF); break;
+        case 128: if constexpr (NNMAX >= 128) multi_item<FWD, 128>(g, bw, smf, layout, NNMAX * OSTRF); break;
+        default: if constexpr (NNMAX >= 256) multi_item<FWD, 256>(g, bw, smf, layout, NNMAX * OSTRF); break;
         }
     }
 }
@@ -989,15 +1089,18 @@ hipError_t launch_dwt_batch_multi(bool fwd, const BatchMultiGroup* groups, const
                                   int grid, hipStream_t st)
 {
     if (nitems < 1 || grid < 1) return hipErrorInvalidValue;
-    const size_t smem = fwd ? 0 : sizeof(float) * (size_t)MULTI_WPB * nnmax * OSTRF;
-    if (smem > lds_limit()) return hipErrorInvalidConfiguration;
     const dim3 g((unsigned)grid), b(64 * MULTI_WPB);
+    // the inverse's per-wave output regions are sized for the kernel's NNM
+    // (every item of the launch uses the same regions)
 #define PFS_M(NNM)                                                                                           \
-    if (nnmax <= NNM)                                                                                        \
+    if (nnmax <= NNM) {                                                                                      \
+        const size_t smem = fwd ? 0 : sizeof(float) * (size_t)MULTI_WPB * NNM * OSTRF;                       \
+        if (smem > lds_limit()) return hipErrorInvalidConfiguration;                                         \
         return fwd ? pfs::launch((k_dwt_batch_multi<true, NNM>), g, b, smem, st, groups, items, nitems,     \
                                  g_mfma_f32_layout)                                                         \
                    : pfs::launch((k_dwt_batch_multi<false, NNM>), g, b, smem, st, groups, items, nitems,    \
-                                 g_mfma_f32_layout);
+                                 g_mfma_f32_layout);                                                        \
+    }
     PFS_M(32) PFS_M(64) PFS_M(128) PFS_M(256)
 #undef PFS_M
     return hipErrorInvalidValue;
