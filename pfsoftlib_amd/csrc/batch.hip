@@ -557,6 +557,41 @@ __device__ __forceinline__ void inv_contract(v4f (&acc)[NKB][SPW], const InvStag
     }
 }
 
+// inv_contract (FP32 tiles) with the chunk's A operands read from the
+// workgroup's LDS copy of its tiles (lt: this lane's float4 of tile 0; tile
+// kb at lt + 256 kb), one group of four tiles ahead of its MFMAs.
+template <int NKB, int SPW>
+__device__ __forceinline__ void inv_contract_lds(v4f (&acc)[NKB][SPW], const InvStage<SPW>& S, const float* lt)
+{
+    float b[SPW][4];
+#pragma unroll
+    for (int sp = 0; sp < SPW; ++sp)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[sp][u] = S.v[sp][u] * S.f[u];
+    constexpr int GS = NKB < 4 ? NKB : 4;
+    v4f a[GS], an[GS];
+#pragma unroll
+    for (int k4 = 0; k4 < GS; ++k4) a[k4] = *reinterpret_cast<const v4f*>(lt + k4 * 256);
+#pragma unroll
+    for (int g = 0; g < NKB / GS; ++g) {
+        if (g + 1 < NKB / GS) {
+#pragma unroll
+            for (int k4 = 0; k4 < GS; ++k4) an[k4] = *reinterpret_cast<const v4f*>(lt + (GS * (g + 1) + k4) * 256);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k4 = 0; k4 < GS; ++k4)
+#pragma unroll
+                for (int sp = 0; sp < SPW; ++sp) {
+                    const int kb = GS * g + k4;
+                    acc[kb][sp] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[k4][u], b[sp][u], acc[kb][sp], 0, 0, 0);
+                }
+#pragma unroll
+        for (int k4 = 0; k4 < GS; ++k4) a[k4] = an[k4];
+    }
+}
+
 // Inverse contraction of one canonical pair for SPW signals per wave:
 // P[k][j] = sum_r D[r][k] * c_s(M + r, case j/2).(re|im by j&1) * sign * norm,
 // A = column-major table tiles (k rows, r columns), B = the coefficients,
@@ -614,34 +649,48 @@ __device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* _
     // make the compiler drain all outstanding loads.
     InvStage<SPW> bc, bn;
     InvTab<NKB, PREC> tab;
+    // staged coefficients with FP32 tiles: the chunk's tiles are staged per
+    // workgroup too when both buffer pairs fit in the output staging (NKB /
+    // WPB 16-byte loads per thread instead of NKB per lane, under the same
+    // per-chunk barrier).  Measured (interleaved A/B, two runs each, same
+    // results): 64 signals at 2B = 256 inverse 3.41-3.43 -> 3.10-3.11 ms,
+    // 512 signals at 2B = 128 2.83-2.84 -> 2.58 ms.
+    constexpr bool LDS_TILES = GATHER == 2 && PREC == 0 && NKB % WPB == 0 &&
+                               16 * 8 * 2 * SG * 2 + 2 * NKB * 256 <= WPB * NN * OSTRF;
+    if constexpr (!LDS_TILES) {
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) inv_tab_load<NKB, PREC>(tab, kb, ti, ts, toff);
+        for (int kb = 0; kb < NKB; ++kb) inv_tab_load<NKB, PREC>(tab, kb, ti, ts, toff);
+    }
     if constexpr (GATHER == 2) {
         // loader role: items e = tid + 64 WPB j (j < SPW) of the chunk's
         // 16 x 8 x SG / 2 16-byte loads: (degree rl, case cl, signal pair h)
         constexpr int CF = 16 * 8 * 2 * SG;
         const int tid = threadIdx.x, sbase = bw.sg * SG;
-        int lrl[SPW], lh[SPW];
-        BCase lcb[SPW];
+        // per item: local degree, signal pair, the case's (M, M') in the
+        // index walk, and the case's degree limit (0 for an inactive case)
+        int lrl[SPW], lh[SPW], lmc[SPW], lmpc[SPW], lrr[SPW];
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
             const int e = tid + 64 * WPB * j;
             lh[j] = e % (SG / 2);
             const int rc = e / (SG / 2);
             lrl[j] = rc >> 3;
-            lcb[j] = bcase(rc & 7, M, Mp, NN);
+            const BCase lc = bcase(rc & 7, M, Mp, NN);
+            lmc[j] = lc.Mc;
+            lmpc[j] = lc.Mpc;
+            lrr[j] = lc.active ? R : 0;
         }
         const float* cbase = reinterpret_cast<const float*>(coeffs) + 2 * sbase;
         float4 ld[SPW];
-        auto load = [&](int q) {
+        auto load = [&](int q) __attribute__((always_inline)) {
 #pragma unroll
             for (int j = 0; j < SPW; ++j) {
                 const int r = 16 * q + lrl[j];
-                const long ci = (lcb[j].active && r < R) ? 2 * cstride * coeff_index32(M + r, lcb[j].Mc, lcb[j].Mpc) : 0;
+                const long ci = r < lrr[j] ? 2 * cstride * coeff_index32(M + r, lmc[j], lmpc[j]) : 0;
                 ld[j] = *reinterpret_cast<const float4*>(cbase + ci + 4 * lh[j]);
             }
         };
-        auto put = [&](int buf) {
+        auto put = [&](int buf) __attribute__((always_inline)) {
             float* st = smf + buf * CF;
 #pragma unroll
             for (int j = 0; j < SPW; ++j) {
@@ -653,7 +702,7 @@ __device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* _
         };
         // this lane's operands of chunk q: degrees 16 q + 4 kk + u, its case
         // and part, signals w SPW .. w SPW + SPW - 1
-        auto take = [&](InvStage<SPW>& S, int buf, int q) {
+        auto take = [&](InvStage<SPW>& S, int buf, int q) __attribute__((always_inline)) {
             const float* st = smf + buf * CF;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -665,18 +714,45 @@ __device__ __forceinline__ void inv_item_body(const PlanTables& t, const long* _
                 for (int sp = 0; sp < SPW; ++sp) S.v[sp][u] = src[sp];
             }
         };
+        // LDS_TILES: chunk q's NKB tiles (NKB x 256 floats, contiguous per
+        // pair) as NKB / WPB float4 per thread, into tile buffer q & 1
+        constexpr int TF = NKB * 256, TJ = LDS_TILES ? NKB / WPB : 1;
+        const float* tsrc = Ti + (roff[bw.pi] >> 4) * NKB * 256;
+        float* tb = smf + 2 * CF;
+        v4f tl[TJ];
+        auto tload = [&](int q) __attribute__((always_inline)) {
+#pragma unroll
+            for (int j = 0; j < TJ; ++j)
+                tl[j] = *reinterpret_cast<const v4f*>(tsrc + (long)q * TF + 4 * (tid + 64 * WPB * j));
+        };
+        auto tput = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) *reinterpret_cast<v4f*>(tb + buf * TF + 4 * (tid + 64 * WPB * j)) = tl[j];
+        };
         load(0);
+        if constexpr (LDS_TILES) tload(0);
         // a workgroup's waves may still be reading a previous item's output
         // staging, which the chunk buffers alias
         __syncthreads();
         put(0);
+        if constexpr (LDS_TILES) tput(0);
         for (int q = 0; q < nch; ++q) {
-            if (q + 1 < nch) load(q + 1);
+            if (q + 1 < nch) {
+                load(q + 1);
+                if constexpr (LDS_TILES) tload(q + 1);
+            }
             __syncthreads();  // chunk q staged by every thread; chunk q - 1's reads done
             take(bc, q & 1, q);
-            const int qn = q + 1 < nch ? q + 1 : q;
-            inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
-            if (q + 1 < nch) put((q + 1) & 1);
+            if constexpr (LDS_TILES) {
+                inv_contract_lds<NKB, SPW>(acc, bc, tb + (q & 1) * TF + 16 * i + 4 * kk);
+            } else {
+                const int qn = q + 1 < nch ? q + 1 : q;
+                inv_contract<NKB, SPW, PREC>(acc, bc, tab, ti + qn * NKB * 256, ts, toff + (long)qn * NKB * 256);
+            }
+            if (q + 1 < nch) {
+                put((q + 1) & 1);
+                if constexpr (LDS_TILES) tput((q + 1) & 1);
+            }
         }
         __syncthreads();  // the last chunk's reads done before the output staging reuses the region
         if (s0 >= nsig) return;
